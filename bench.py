@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the ADI time step on one MI355X.
+
+Workload: the 256^3 irregular domain with 40 seeded ellipsoids removed (FP32,
+inlet u=1, outlet BC_FREE, Re=1000, dt=2e-4, 4 Picard iterations, 1 local
+sweep).  A step is UpdateBoundaries + TimeStep over the whole grid, with the
+state resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|plain]
+
+Prints one JSON line: cell-updates/s of the headline path, the roofline of the
+dominant kernel (the sweep: algorithmic bytes (16w+1)*S_d per sweep over the
+measured sweep-kernel time from HIP events on the solver's stream), the plain
+per-axis-launch path for comparison, and the oracle (C++/OpenMP restatement of
+the reference CPU path) timed on a bounded sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cmc-fluid-solver_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "ADI cell-updates/s + %HBM peak, batched vs plain on 1 GPU; adjoint/forward time ratio"
+
+
+def _sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32"):
+    from cmcx.solver import make_solver
+
+    s = make_solver(dom, dtype, path=path)
+    G, L = dom.num_global, dom.num_local
+    for _ in range(warmup):
+        s.UpdateBoundaries()
+        s.TimeStep(dom.dt, G, L, False)
+    s.synchronize()
+    _sync()
+    # kernel timing pass (HIP events around each sweep launch on the solver's stream)
+    s.profile(True)
+    s.profile_read(reset=True)
+    for _ in range(steps):
+        s.UpdateBoundaries()
+        s.TimeStep(dom.dt, G, L, False)
+    sweep_ms, launches = s.profile_read(reset=True)
+    s.profile(False)
+    s.synchronize()
+    _sync()
+    # wall-clock pass without events
+    t0 = time.perf_counter()
+    for i in range(steps):
+        s.UpdateBoundaries()
+        s.TimeStep(dom.dt, G, L, i == steps - 1)
+    s.synchronize()
+    _sync()
+    wall = time.perf_counter() - t0
+    step_bytes = s.step_bytes(G, L)
+    err = s.div_error()
+    s.close()
+    return {
+        "ms_per_step": wall / steps * 1e3,
+        "cell_updates_per_s": dom.n * steps / wall,
+        "sweep_ms_per_launch": sweep_ms / max(launches, 1),
+        "sweep_launches": launches,
+        "sweep_bytes_per_step": step_bytes,
+        "sweep_achieved_gbs": step_bytes * steps / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0,
+        "div_error": err,
+    }
+
+
+def cpu_baseline(dom, budget_s: float = 20.0):
+    """Oracle (C++/OpenMP port of the reference CPU path) on a bounded sample:
+    whole FP32 time steps of the same workload until `budget_s` is spent."""
+    import oracle_lib
+
+    L = oracle_lib.load()
+    threads = int(L.orc_max_threads())
+    o = oracle_lib.Oracle3D(dom, "f32")
+    o.create_segments()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.update_boundaries()
+        o.step(dom.dt, dom.num_global, dom.num_local, False)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    wall = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": dom.n * n / wall,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} full FP32 time step(s) of the same 256^3 workload (G=4, L=1), {wall:.1f} s, OpenMP",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", choices=["fused", "plain"], default=None)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs the whole domain on a single "
+              "MI355X (no multi-GPU path); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from cmcx import domains
+    from cmcx._lib import CMCX_PATH_FUSED, CMCX_PATH_PLAIN, lib
+
+    lib()
+    dom = domains.ellipsoids(args.size)
+    fused_ok = True
+    try:
+        from cmcx.solver import make_solver
+
+        probe = make_solver(domains.box(8, 8, 8), "f32", path=CMCX_PATH_FUSED)
+        probe.close()
+    except Exception:
+        fused_ok = False
+    head_path = args.path or ("fused" if fused_ok else "plain")
+    head = time_path(dom, CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN,
+                     args.steps, args.warmup)
+    plain = None
+    if head_path == "fused" and not args.no_plain:
+        plain = time_path(dom, CMCX_PATH_PLAIN, max(2, args.steps // 4), 1)
+    cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
+
+    achieved = head["sweep_achieved_gbs"]
+    line = {
+        "metric": METRIC,
+        "value": head["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ellipsoids_{args.size}^3_fp32",
+            "grid": [dom.nx, dom.ny, dom.nz],
+            "ellipsoids": 40,
+            "seed": 0,
+            "solid_fraction": round(dom.solid_fraction(), 4),
+            "Re": dom.re,
+            "dt": dom.dt,
+            "num_global": dom.num_global,
+            "num_local": dom.num_local,
+            "path": head_path,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "sweep",
+            "bytes_per_step": head["sweep_bytes_per_step"],
+            "sweep_ms_per_launch": head["sweep_ms_per_launch"],
+        },
+        "cpu_baseline": cpu,
+    }
+    if plain is not None:
+        line["plain"] = {
+            "value": plain["cell_updates_per_s"],
+            "ms_per_step": plain["ms_per_step"],
+            "sweep_achieved_gbs": plain["sweep_achieved_gbs"],
+            "speedup_fused_over_plain": head["cell_updates_per_s"] / plain["cell_updates_per_s"],
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

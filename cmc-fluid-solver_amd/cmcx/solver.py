@@ -1,0 +1,215 @@
+"""Host-side mirror of the reference's solver interface over the C ABI.
+
+``AdiSolver3D`` keeps the method names, argument meaning and error behaviour
+of src/FluidSolver3D/AdiSolver3D.h:49-114 and Solver3D.h:24-49:
+
+    solver = AdiSolver3D()
+    solver.Init(GPU, False, grid, params)       # AdiSolver3D::Init
+    solver.CreateSegments()                     # AdiSolver3D::CreateSegments
+    for i in range(steps):
+        solver.UpdateBoundaries()               # AdiSolver3D::UpdateBoundaries
+        solver.TimeStep(dt, G, L, i % 10 == 0)  # AdiSolver3D::TimeStep, raises DivergenceError
+    vel, T = solver.GetLayer()                  # Solver3D::GetLayer (lags one step, as the reference)
+
+``grid`` is a :class:`cmcx.domains.Domain3D` (node arrays of the reference's
+Grid3D).  The only backend is the GPU; asking for the CPU backend raises.
+Beyond the reference interface, :meth:`fields` returns the current state
+(u, v, w, p, T) with p = v_T * T.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .params import FluidParams
+
+CPU, GPU = 0, 1
+X, Y, Z = 0, 1, 2
+
+_NP = {_lib.CMCX_F32: np.float32, _lib.CMCX_F64: np.float64}
+
+
+def _u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class AdiSolver3D:
+    def __init__(self, dtype: str = "f32"):
+        self.h = ctypes.c_void_p(None)
+        self.dtype = _lib.CMCX_F32 if dtype in ("f32", "float32", np.float32) else _lib.CMCX_F64
+        self.np_dtype = _NP[self.dtype]
+        self.grid = None
+        self.params = None
+        self._keep = []
+
+    # -- reference interface -------------------------------------------------------
+    def Init(self, backend, csv, grid, params: FluidParams, useBlocking: bool = False,
+             nblockZ: int = 1, stream=None):
+        """AdiSolver3D::Init (AdiSolver3D.cpp:166-268)."""
+        if backend != GPU:
+            raise ValueError("only the GPU backend exists (no CPU fallback)")
+        if useBlocking:
+            raise NotImplementedError("the Z-blocked multi-GPU X/Y pipeline is not built (one GPU)")
+        L = lib()
+        self.grid = grid
+        self.params = params
+        arrs = [_u8(grid.type), _u8(grid.bc_vel), _u8(grid.bc_temp)]
+        vals = [_f64(grid.vx), _f64(grid.vy), _f64(grid.vz), _f64(grid.T)]
+        self._keep = arrs + vals
+        dom = _lib.cmcx_domain()
+        dom.dims[:] = [grid.nx, grid.ny, grid.nz]
+        dom.h[:] = [grid.dx, grid.dy, grid.dz]
+        dom.base_t = grid.base_t
+        dom.type, dom.bc_vel, dom.bc_temp = (a.ctypes.data_as(_lib._u8p) for a in arrs)
+        for c in range(3):
+            dom.v[c] = vals[c].ctypes.data_as(_lib._dp)
+        dom.T = vals[3].ctypes.data_as(_lib._dp)
+        prm = _lib.cmcx_params(params.v_T, params.v_vis, params.t_vis, params.t_phi)
+        h = ctypes.c_void_p(None)
+        check(L.cmcx_create(ctypes.byref(dom), ctypes.byref(prm), self.dtype,
+                            ctypes.c_void_p(stream), ctypes.byref(h)), "cmcx_create")
+        self.h = h
+        if nblockZ != 1:
+            check(L.cmcx_set_nblockz(self.h, nblockZ), "cmcx_set_nblockz")
+        return self
+
+    def SetOptionsGPU(self, transpose: bool = False, decompose: bool = False):
+        """AdiSolver3D::SetOptionsGPU.  `transpose` (coalesced Z sweeps in the
+        reference) selects the fused line-tiled sweeps, which stage every axis
+        through LDS; `decompose` is parsed and unused, as in the reference."""
+        self.set_options(path=_lib.CMCX_PATH_FUSED if transpose else _lib.CMCX_PATH_PLAIN)
+
+    def CreateSegments(self):
+        check(lib().cmcx_build_segments(self.h), "cmcx_build_segments")
+
+    def UpdateBoundaries(self):
+        check(lib().cmcx_update_boundaries(self.h), "cmcx_update_boundaries")
+
+    def TimeStep(self, dt: float, num_global: int, num_local: int, computeError: bool) -> float:
+        """AdiSolver3D::TimeStep; raises DivergenceError like the reference's throw."""
+        err = ctypes.c_double(0.0)
+        check(lib().cmcx_step(self.h, float(dt), int(num_global), int(num_local),
+                              1 if computeError else 0, ctypes.byref(err)), "cmcx_step")
+        return err.value
+
+    def GetLayer(self, outdimx: int = 0, outdimy: int = 0, outdimz: int = 0):
+        """Solver3D::GetLayer: (vel[n,3], T[n]) of the layer the reference outputs."""
+        g = self.grid
+        ox, oy, oz = outdimx or g.nx, outdimy or g.ny, outdimz or g.nz
+        vel = np.zeros((ox * oy * oz, 3), self.np_dtype)
+        T = np.zeros(ox * oy * oz, np.float64)
+        check(lib().cmcx_get_layer(self.h, vel.ctypes.data, T.ctypes.data_as(_lib._dp), ox, oy, oz),
+              "cmcx_get_layer")
+        return vel, T
+
+    def sum_layer(self, ch) -> float:
+        return 0.0  # the reference returns 0 (AdiSolver3D.cpp:30-58)
+
+    def debug(self, flag: bool):
+        pass
+
+    # -- extensions --------------------------------------------------------------------
+    def set_options(self, path: int = _lib.CMCX_PATH_PLAIN, merge_order: int = _lib.CMCX_MERGE_CPU):
+        check(lib().cmcx_set_options(self.h, path, merge_order), "cmcx_set_options")
+
+    def num_segments(self, axis: int) -> int:
+        n = ctypes.c_int64(0)
+        check(lib().cmcx_num_segments(self.h, axis, ctypes.byref(n)), "cmcx_num_segments")
+        return n.value
+
+    def segments(self, axis: int) -> np.ndarray:
+        n = self.num_segments(axis)
+        out = np.zeros((max(n, 1), 7), np.int32)
+        check(lib().cmcx_get_segments(self.h, axis, out.ctypes.data_as(_lib._i32p), n), "cmcx_get_segments")
+        return out[:n]
+
+    def segment_nodes(self, axis: int):
+        n = self.num_segments(axis)
+        flags = np.zeros((max(n, 1), 6), np.uint8)
+        vals = np.zeros((max(n, 1), 8), np.float64)
+        check(lib().cmcx_get_segment_nodes(self.h, axis, flags.ctypes.data_as(_lib._u8p),
+                                           vals.ctypes.data_as(_lib._dp), n), "cmcx_get_segment_nodes")
+        return flags[:n], vals[:n]
+
+    def fields(self):
+        """Current state (u, v, w, p, T) as numpy arrays of the field type."""
+        n = self.grid.n
+        out = [np.zeros(n, self.np_dtype) for _ in range(5)]
+        check(lib().cmcx_get_fields(self.h, *[a.ctypes.data for a in out]), "cmcx_get_fields")
+        return tuple(out)
+
+    def set_fields(self, u, v, w, T):
+        arrs = [np.ascontiguousarray(a, dtype=self.np_dtype) for a in (u, v, w, T)]
+        check(lib().cmcx_set_fields(self.h, *[a.ctypes.data for a in arrs]), "cmcx_set_fields")
+
+    def fields_device(self, u, v, w, p, T):
+        """Copy the state into device buffers given as integer addresses (e.g. torch data_ptr())."""
+        check(lib().cmcx_get_fields_device(self.h, u, v, w, p, T), "cmcx_get_fields_device")
+
+    def state_ptrs(self):
+        arr = (ctypes.c_void_p * 4)()
+        check(lib().cmcx_state_device_ptrs(self.h, arr), "cmcx_state_device_ptrs")
+        return [arr[i] for i in range(4)]
+
+    def div_error(self) -> float:
+        e = ctypes.c_double(0.0)
+        check(lib().cmcx_div_error(self.h, ctypes.byref(e)), "cmcx_div_error")
+        return e.value
+
+    def run(self, steps: int, dt: float, num_global: int = 4, num_local: int = 1) -> float:
+        e = ctypes.c_double(0.0)
+        check(lib().cmcx_run(self.h, float(dt), steps, num_global, num_local, ctypes.byref(e)), "cmcx_run")
+        return e.value
+
+    def set_boundary(self, bc_vel, bc_temp, vx, vy, vz, T):
+        a = [_u8(bc_vel), _u8(bc_temp)]
+        v = [_f64(vx), _f64(vy), _f64(vz), _f64(T)]
+        check(lib().cmcx_set_boundary(self.h, *[x.ctypes.data_as(_lib._u8p) for x in a],
+                                      *[x.ctypes.data_as(_lib._dp) for x in v]), "cmcx_set_boundary")
+
+    def profile(self, enable: bool):
+        check(lib().cmcx_profile(self.h, 1 if enable else 0), "cmcx_profile")
+
+    def profile_read(self, reset: bool = True):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        check(lib().cmcx_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n), 1 if reset else 0),
+              "cmcx_profile_read")
+        return ms.value, n.value
+
+    def step_bytes(self, num_global: int, num_local: int = 1) -> float:
+        b = ctypes.c_double(0.0)
+        check(lib().cmcx_step_bytes(self.h, num_global, num_local, ctypes.byref(b)), "cmcx_step_bytes")
+        return b.value
+
+    def synchronize(self):
+        check(lib().cmcx_synchronize(self.h), "cmcx_synchronize")
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().cmcx_destroy(self.h)
+            self.h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_solver(dom, dtype: str = "f32", path: int = _lib.CMCX_PATH_PLAIN,
+                merge_order: int = _lib.CMCX_MERGE_CPU, params: FluidParams | None = None,
+                nblockz: int = 1) -> AdiSolver3D:
+    """Init + SetOptions + CreateSegments for a domain in one call."""
+    s = AdiSolver3D(dtype)
+    s.Init(GPU, False, dom, params or dom.params(), nblockZ=nblockz)
+    s.set_options(path, merge_order)
+    s.CreateSegments()
+    return s

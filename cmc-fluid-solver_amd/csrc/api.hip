@@ -1,0 +1,650 @@
+// C ABI and host orchestration of the ADI time step.
+//
+// One solver object owns the device copy of the domain (mask, boundary kinds,
+// node values), five field layers and the segment lists.  Layer roles:
+//   cur   state at the start of the step (TimeLayer3D `cur`)
+//   next  output of the Z and X sweeps (`next`)
+//   half  output of the Y sweep (`half`)
+//   tA,tB the Picard layer `temp`, double-buffered: every sweep reads one buffer
+//         and writes the merged values of its NODE_IN nodes into the other, so
+//         the four fields of a sweep can be solved in one launch while the
+//         cross-line stencils still see the pre-sweep layer (the reference's CPU
+//         order, AdiSolver3D.cpp:591-604 then 640-653).
+// TimeStep's extra merge after each X sweep (AdiSolver3D.cpp:347-357) is folded
+// into the X sweep of the last local iteration (merge applied twice).
+#include "cmcx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+namespace cmcx {
+
+thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+struct ISolver {
+    virtual ~ISolver() {}
+    virtual int set_options(int path, int merge_order) = 0;
+    virtual int set_nblockz(int nb) = 0;
+    virtual int build_segments() = 0;
+    virtual int num_segments(int axis, int64_t* n) = 0;
+    virtual int get_segments(int axis, int32_t* out, int64_t cap) = 0;
+    virtual int get_segment_nodes(int axis, uint8_t* flags, double* vals, int64_t cap) = 0;
+    virtual int update_boundaries() = 0;
+    virtual int step(double dt, int G, int L, int compute_err, double* err) = 0;
+    virtual int div_error(double* err) = 0;
+    virtual int get_fields(void* u, void* v, void* w, void* p, void* T, bool device) = 0;
+    virtual int set_fields(const void* u, const void* v, const void* w, const void* T) = 0;
+    virtual int state_ptrs(void** p4) = 0;
+    virtual int get_layer(void* vel, double* T, int ox, int oy, int oz) = 0;
+    virtual int set_boundary(const uint8_t* bv, const uint8_t* bt, const double* vx,
+                             const double* vy, const double* vz, const double* T) = 0;
+    virtual int profile(int enable) = 0;
+    virtual int profile_read(double* ms, int64_t* launches, int reset) = 0;
+    virtual int step_bytes(int G, int L, double* bytes) = 0;
+    virtual int synchronize() = 0;
+};
+
+template <class T>
+class Solver final : public ISolver {
+   public:
+    Geom g{};
+    double hx_d, hy_d, hz_d, base_t;
+    cmcx_params prm{};
+    hipStream_t st = nullptr;
+    int path = CMCX_PATH_PLAIN;
+    int merge_order = CMCX_MERGE_CPU;
+    int nblockz = 1;
+    bool built = false;
+    double diff_error = 0.0;
+
+    uint8_t *d_type = nullptr, *d_bcv = nullptr, *d_bct = nullptr;
+    T* d_g[4] = {nullptr, nullptr, nullptr, nullptr};
+    T* buf = nullptr;  // 5 layers x 4 fields x N
+    Layer4<T> cur{}, next{}, half{}, tA{}, tB{};
+    SegList segs[3];
+    T* scratch = nullptr;
+    size_t scratch_elems = 0;
+    int32_t* lines[3] = {nullptr, nullptr, nullptr};
+    int64_t nlines[3] = {0, 0, 0};
+    double* d_acc = nullptr;
+    // profiling
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+
+    Solver(const cmcx_domain* d, const cmcx_params* p, hipStream_t stream) {
+        st = stream;
+        g.nx = d->dims[0];
+        g.ny = d->dims[1];
+        g.nz = d->dims[2];
+        g.sy = g.nz;
+        g.sx = (int64_t)g.ny * g.nz;
+        g.N = g.sx * g.nx;
+        hx_d = d->h[0];
+        hy_d = d->h[1];
+        hz_d = d->h[2];
+        base_t = d->base_t;
+        prm = *p;
+        const int64_t N = g.N;
+        CMCX_HIP(hipMalloc((void**)&d_type, N));
+        CMCX_HIP(hipMalloc((void**)&d_bcv, N));
+        CMCX_HIP(hipMalloc((void**)&d_bct, N));
+        for (int f = 0; f < 4; f++) CMCX_HIP(hipMalloc((void**)&d_g[f], sizeof(T) * N));
+        CMCX_HIP(hipMalloc((void**)&buf, sizeof(T) * N * 20));
+        CMCX_HIP(hipMemsetAsync(buf, 0, sizeof(T) * N * 20, st));
+        Layer4<T>* Ls[5] = {&cur, &next, &half, &tA, &tB};
+        for (int l = 0; l < 5; l++)
+            for (int f = 0; f < 4; f++) Ls[l]->f[f] = buf + (int64_t)(l * 4 + f) * N;
+        CMCX_HIP(hipMalloc((void**)&d_acc, sizeof(double) * 2));
+        CMCX_HIP(hipMemcpyAsync(d_type, d->type, N, hipMemcpyHostToDevice, st));
+        upload_boundary(d->bc_vel, d->bc_temp, d->v[0], d->v[1], d->v[2], d->T);
+        // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
+        for (int f = 0; f < 4; f++)
+            CMCX_HIP(hipMemcpyAsync(cur.f[f], d_g[f], sizeof(T) * N, hipMemcpyDeviceToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+    }
+
+    void upload_boundary(const uint8_t* bv, const uint8_t* bt, const double* vx, const double* vy,
+                         const double* vz, const double* Tt) {
+        const int64_t N = g.N;
+        CMCX_HIP(hipMemcpyAsync(d_bcv, bv, N, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMemcpyAsync(d_bct, bt, N, hipMemcpyHostToDevice, st));
+        std::vector<T> tmp(N);
+        const double* src[4] = {vx, vy, vz, Tt};
+        for (int f = 0; f < 4; f++) {
+            for (int64_t i = 0; i < N; i++) tmp[i] = (T)src[f][i];
+            CMCX_HIP(hipMemcpyAsync(d_g[f], tmp.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+            CMCX_HIP(hipStreamSynchronize(st));  // tmp is reused
+        }
+    }
+
+    ~Solver() override {
+        hipStreamSynchronize(st);
+        hipFree(d_type);
+        hipFree(d_bcv);
+        hipFree(d_bct);
+        for (auto* p : d_g) hipFree(p);
+        hipFree(buf);
+        for (auto& s : segs) hipFree(s.d);
+        hipFree(scratch);
+        for (auto* l : lines) hipFree(l);
+        hipFree(d_acc);
+        for (auto& e : ev_pool) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+    }
+
+    int check_mask() {
+        unsigned long long* d_out = nullptr;
+        unsigned long long h[2] = {0, 0};
+        CMCX_HIP(hipMalloc((void**)&d_out, sizeof(h)));
+        CMCX_HIP(hipMemsetAsync(d_out, 0, sizeof(h), st));
+        launch_mask_check(g, d_type, d_out, st);
+        CMCX_HIP(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+        CMCX_HIP(hipFree(d_out));
+        if (h[0] || h[1]) {
+            set_error("mask not well formed: " + std::to_string(h[0]) +
+                      " NODE_IN nodes on a box face, " + std::to_string(h[1]) +
+                      " NODE_OUT nodes next to NODE_IN");
+            return CMCX_ERR_MASK;
+        }
+        return CMCX_OK;
+    }
+
+    int set_options(int p, int m) override {
+        if (p != CMCX_PATH_PLAIN && p != CMCX_PATH_FUSED) return CMCX_ERR_ARG;
+        if (m != CMCX_MERGE_CPU && m != CMCX_MERGE_GPU_ORDER) return CMCX_ERR_ARG;
+        if (p == CMCX_PATH_FUSED && m != CMCX_MERGE_CPU) {
+            set_error("the fused path implements the CPU merge order only");
+            return CMCX_ERR_UNSUPPORTED;
+        }
+        if (p == CMCX_PATH_FUSED && !fused_supported<T>(g)) {
+            set_error("fused path: grid lines longer than supported");
+            return CMCX_ERR_UNSUPPORTED;
+        }
+        path = p;
+        merge_order = m;
+        if (built) prepare_path_buffers();
+        return CMCX_OK;
+    }
+
+    int set_nblockz(int nb) override {
+        if (nb < 1 || nb > g.nz || nb > 64) return CMCX_ERR_ARG;
+        nblockz = nb;
+        built = false;
+        return CMCX_OK;
+    }
+
+    void prepare_path_buffers() {
+        if (path == CMCX_PATH_PLAIN) {
+            size_t need = 0;
+            for (auto& s : segs) need = std::max(need, (size_t)s.max_size * 4 * (size_t)s.n * 2);
+            if (need > scratch_elems) {
+                hipFree(scratch);
+                scratch = nullptr;
+                CMCX_HIP(hipMalloc((void**)&scratch, sizeof(T) * need));
+                scratch_elems = need;
+            }
+        } else {
+            for (int a = 0; a < 3; a++)
+                if (!lines[a]) build_active_lines(d_type, g, a, &lines[a], &nlines[a], st);
+        }
+    }
+
+    int build_segments() override {
+        for (int a = 0; a < 3; a++) build_segment_list(d_type, g, a, nblockz, segs[a], st);
+        prepare_path_buffers();
+        built = true;
+        return CMCX_OK;
+    }
+
+    int num_segments(int axis, int64_t* n) override {
+        if (axis < 0 || axis > 2 || !n) return CMCX_ERR_ARG;
+        if (!built) return CMCX_ERR_STATE;
+        *n = segs[axis].n;
+        return CMCX_OK;
+    }
+
+    int get_segments(int axis, int32_t* out, int64_t cap) override {
+        if (axis < 0 || axis > 2 || !out) return CMCX_ERR_ARG;
+        if (!built) return CMCX_ERR_STATE;
+        const SegList& s = segs[axis];
+        if (cap < s.n) return CMCX_ERR_ARG;
+        if (s.n == 0) return CMCX_OK;
+        std::vector<int32_t> h(7 * s.n);
+        CMCX_HIP(hipMemcpyAsync(h.data(), s.d, sizeof(int32_t) * 7 * s.n, hipMemcpyDeviceToHost, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < s.n; i++)
+            for (int c = 0; c < 7; c++) out[7 * i + c] = h[c * s.n + i];
+        return CMCX_OK;
+    }
+
+    int get_segment_nodes(int axis, uint8_t* flags, double* vals, int64_t cap) override {
+        if (axis < 0 || axis > 2 || !flags || !vals) return CMCX_ERR_ARG;
+        if (!built) return CMCX_ERR_STATE;
+        int64_t n = segs[axis].n;
+        if (cap < n) return CMCX_ERR_ARG;
+        std::vector<int32_t> sl(7 * std::max<int64_t>(n, 1));
+        if (n) {
+            std::vector<int32_t> h(7 * n);
+            CMCX_HIP(hipMemcpyAsync(h.data(), segs[axis].d, sizeof(int32_t) * 7 * n,
+                                    hipMemcpyDeviceToHost, st));
+            CMCX_HIP(hipStreamSynchronize(st));
+            for (int64_t i = 0; i < n; i++)
+                for (int c = 0; c < 7; c++) sl[7 * i + c] = h[c * n + i];
+        }
+        // node records from the device copy of the grid
+        const int64_t N = g.N;
+        std::vector<uint8_t> t(N), bv(N), bt(N);
+        std::vector<T> gv[4];
+        CMCX_HIP(hipMemcpyAsync(t.data(), d_type, N, hipMemcpyDeviceToHost, st));
+        CMCX_HIP(hipMemcpyAsync(bv.data(), d_bcv, N, hipMemcpyDeviceToHost, st));
+        CMCX_HIP(hipMemcpyAsync(bt.data(), d_bct, N, hipMemcpyDeviceToHost, st));
+        for (int f = 0; f < 4; f++) {
+            gv[f].resize(N);
+            CMCX_HIP(hipMemcpyAsync(gv[f].data(), d_g[f], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+        }
+        CMCX_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < n; i++) {
+            const int32_t* q = &sl[7 * i];
+            int64_t ids[2] = {q[0] * g.sx + q[1] * g.sy + q[2], q[3] * g.sx + q[4] * g.sy + q[5]};
+            for (int e = 0; e < 2; e++) {
+                int64_t id = ids[e];
+                flags[6 * i + 3 * e + 0] = t[id];
+                flags[6 * i + 3 * e + 1] = bv[id];
+                flags[6 * i + 3 * e + 2] = bt[id];
+                for (int f = 0; f < 4; f++) vals[8 * i + 4 * e + f] = (double)gv[f][id];
+            }
+        }
+        return CMCX_OK;
+    }
+
+    int update_boundaries() override {
+        launch_update_boundaries<T>(g, d_type, d_g, cur, st);
+        return CMCX_OK;
+    }
+
+    SweepParams<T> sweep_params(int axis, T dt) const {
+        SweepParams<T> p;
+        p.dt = dt;
+        p.hx = (T)hx_d;
+        p.hy = (T)hy_d;
+        p.hz = (T)hz_d;
+        p.h = axis == AX_X ? p.hx : axis == AX_Y ? p.hy : p.hz;
+        T v_vis = (T)prm.v_vis, t_vis = (T)prm.t_vis;
+        p.vis_v = v_vis / (p.h * p.h);
+        p.vis_t = t_vis / (p.h * p.h);
+        p.v_T = (T)prm.v_T;
+        p.t_phi = (T)prm.t_phi;
+        return p;
+    }
+
+    hipEvent_t* next_events() {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t a, b;
+            CMCX_HIP(hipEventCreate(&a));
+            CMCX_HIP(hipEventCreate(&b));
+            ev_pool.push_back({a, b});
+        }
+        return &ev_pool[ev_used++].first;
+    }
+
+    void harvest_events() {
+        if (!ev_used) return;
+        CMCX_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < ev_used; i++) {
+            float ms = 0.f;
+            CMCX_HIP(hipEventElapsedTime(&ms, ev_pool[i].first, ev_pool[i].second));
+            prof_ms += ms;
+        }
+        prof_launches += (int64_t)ev_used;
+        ev_used = 0;
+    }
+
+    void sweep(int axis, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
+               int var0, int var1, const SweepParams<T>& p) {
+        hipEvent_t* ev = nullptr;
+        if (prof) {
+            if (ev_used >= 4096) harvest_events();
+            ev = next_events();
+            CMCX_HIP(hipEventRecord(ev[0], st));
+        }
+        if (path == CMCX_PATH_PLAIN)
+            launch_plain_sweep<T>(g, axis, segs[axis], p, d_bcv, d_bct, d_g, in, tin, out, tout,
+                                  merges, var0, var1, scratch, st);
+        else
+            launch_fused_sweep<T>(g, axis, p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout, merges,
+                                  lines[axis], nlines[axis], st);
+        if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
+    }
+
+    int step(double dt_d, int G, int L, int compute_err, double* err) override {
+        if (!built) {
+            set_error("step before build_segments");
+            return CMCX_ERR_STATE;
+        }
+        if (G < 0 || L < 1) return CMCX_ERR_ARG;
+        const T dt = (T)dt_d;
+        SweepParams<T> P[3] = {sweep_params(AX_X, dt), sweep_params(AX_Y, dt), sweep_params(AX_Z, dt)};
+        if (merge_order == CMCX_MERGE_CPU) {
+            launch_prep_step<T>(g, d_type, cur, next, tA, tB, st);
+            Layer4<T> tin = cur;  // temp == cur at the first sweep
+            auto other = [&](const Layer4<T>& t) { return t.f[0] == tB.f[0] ? tA : tB; };
+            for (int it = 0; it < G; it++) {
+                const int axes[3] = {AX_Z, AX_Y, AX_X};
+                Layer4<T> ins[3] = {cur, next, half};
+                Layer4<T> outs[3] = {next, half, next};
+                for (int q = 0; q < 3; q++) {
+                    int a = axes[q];
+                    for (int l = 0; l < L; l++) {
+                        Layer4<T> tout = other(tin);
+                        int merges = (a == AX_X && l == L - 1) ? 2 : 1;
+                        sweep(a, ins[q], tin, outs[q], tout, merges, 0, 4, P[a]);
+                        tin = tout;
+                    }
+                }
+            }
+        } else {
+            // the reference's CUDA order: temp <- cur, then per field in-place merges
+            Layer4<T> none{};
+            launch_prep_step<T>(g, d_type, cur, next, tA, none, st);
+            for (int it = 0; it < G; it++) {
+                const int axes[3] = {AX_Z, AX_Y, AX_X};
+                Layer4<T> ins[3] = {cur, next, half};
+                Layer4<T> outs[3] = {next, half, next};
+                for (int q = 0; q < 3; q++)
+                    for (int l = 0; l < L; l++)
+                        for (int v = 0; v < 4; v++)
+                            sweep(axes[q], ins[q], tA, outs[q], tA, 1, v, v + 1, P[axes[q]]);
+            }
+        }
+        if (compute_err) diff_error = eval_div(next);
+        if (err) *err = diff_error;
+        if (prof) harvest_events();
+        if (diff_error > 0.01) {
+            set_error("divergence error above threshold (0.01)");
+            return CMCX_DIVERGED;
+        }
+        std::swap(cur, next);
+        return CMCX_OK;
+    }
+
+    double eval_div(Layer4<T> Lr) {
+        launch_zero_f64(d_acc, 2, st);
+        launch_div_error<T>(g, d_type, Lr, (T)hx_d, (T)hy_d, (T)hz_d, d_acc, st);
+        double h[2];
+        CMCX_HIP(hipMemcpyAsync(h, d_acc, sizeof(h), hipMemcpyDeviceToHost, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+        return h[0] / h[1];
+    }
+
+    int div_error(double* err) override {
+        if (!err) return CMCX_ERR_ARG;
+        *err = eval_div(cur);
+        return CMCX_OK;
+    }
+
+    int get_fields(void* u, void* v, void* w, void* p, void* Tt, bool device) override {
+        const int64_t N = g.N;
+        hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        void* dst[4] = {u, v, w, Tt};
+        for (int f = 0; f < 4; f++)
+            if (dst[f]) CMCX_HIP(hipMemcpyAsync(dst[f], cur.f[f], sizeof(T) * N, k, st));
+        if (p) {
+            if (device) {
+                launch_scale<T>(cur.f[3], (T*)p, (T)prm.v_T, N, st);
+            } else {
+                // p = v_T * T, the ideal-gas closure the reference's momentum rows use
+                std::vector<T> tmp(N);
+                CMCX_HIP(hipMemcpyAsync(tmp.data(), cur.f[3], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+                CMCX_HIP(hipStreamSynchronize(st));
+                T vT = (T)prm.v_T;
+                T* pp = (T*)p;
+                for (int64_t i = 0; i < N; i++) pp[i] = vT * tmp[i];
+            }
+        }
+        CMCX_HIP(hipStreamSynchronize(st));
+        return CMCX_OK;
+    }
+
+    int set_fields(const void* u, const void* v, const void* w, const void* Tt) override {
+        const void* src[4] = {u, v, w, Tt};
+        for (int f = 0; f < 4; f++)
+            if (src[f])
+                CMCX_HIP(hipMemcpyAsync(cur.f[f], src[f], sizeof(T) * g.N, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+        return CMCX_OK;
+    }
+
+    int state_ptrs(void** p4) override {
+        if (!p4) return CMCX_ERR_ARG;
+        for (int f = 0; f < 4; f++) p4[f] = cur.f[f];
+        return CMCX_OK;
+    }
+
+    int get_layer(void* vel, double* Tout, int ox, int oy, int oz) override {
+        if (!vel || !Tout || ox < 0 || oy < 0 || oz < 0) return CMCX_ERR_ARG;
+        if (ox == 0) ox = g.nx;
+        if (oy == 0) oy = g.ny;
+        if (oz == 0) oz = g.nz;
+        // Solver3D::GetLayer: next->Clear(grid, NODE_OUT, 99999, ...) then FilterToArrays
+        T miss = (T)99999.0f;
+        launch_fill_typed<T>(g, d_type, NODE_OUT, next, miss, miss, miss, miss, st);
+        const int64_t N = g.N;
+        std::vector<T> h[4];
+        for (int f = 0; f < 4; f++) {
+            h[f].resize(N);
+            CMCX_HIP(hipMemcpyAsync(h[f].data(), next.f[f], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+        }
+        CMCX_HIP(hipStreamSynchronize(st));
+        T* V = (T*)vel;
+        for (int i = 0; i < ox; i++)
+            for (int j = 0; j < oy; j++)
+                for (int k = 0; k < oz; k++) {
+                    int x = i * g.nx / ox, y = j * g.ny / oy, z = k * g.nz / oz;
+                    int64_t src = x * g.sx + y * g.sy + z;
+                    int64_t ind = ((int64_t)i * oy + j) * oz + k;
+                    V[3 * ind + 0] = h[0][src];
+                    V[3 * ind + 1] = h[1][src];
+                    V[3 * ind + 2] = h[2][src];
+                    Tout[ind] = (double)h[3][src];
+                }
+        return CMCX_OK;
+    }
+
+    int set_boundary(const uint8_t* bv, const uint8_t* bt, const double* vx, const double* vy,
+                     const double* vz, const double* Tt) override {
+        if (!bv || !bt || !vx || !vy || !vz || !Tt) return CMCX_ERR_ARG;
+        upload_boundary(bv, bt, vx, vy, vz, Tt);
+        return CMCX_OK;
+    }
+
+    int profile(int enable) override {
+        if (prof && !enable) harvest_events();
+        prof = enable != 0;
+        return CMCX_OK;
+    }
+
+    int profile_read(double* ms, int64_t* launches, int reset) override {
+        harvest_events();
+        if (ms) *ms = prof_ms;
+        if (launches) *launches = prof_launches;
+        if (reset) {
+            prof_ms = 0.0;
+            prof_launches = 0;
+        }
+        return CMCX_OK;
+    }
+
+    int step_bytes(int G, int L, double* bytes) override {
+        if (!bytes) return CMCX_ERR_ARG;
+        if (!built) return CMCX_ERR_STATE;
+        double S = (double)segs[0].sum_size + segs[1].sum_size + segs[2].sum_size;
+        *bytes = (double)G * L * (16.0 * sizeof(T) + 1.0) * S;
+        return CMCX_OK;
+    }
+
+    int synchronize() override {
+        CMCX_HIP(hipStreamSynchronize(st));
+        return CMCX_OK;
+    }
+};
+
+}  // namespace cmcx
+
+using namespace cmcx;
+
+struct cmcx_solver {
+    int dtype;
+    std::unique_ptr<ISolver> impl;
+};
+
+#define CMCX_GUARD(body)                                                                       \
+    try {                                                                                      \
+        body                                                                                   \
+    } catch (const cmcx::HipError& e) {                                                        \
+        cmcx::set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where); \
+        return CMCX_ERR_HIP;                                                                   \
+    } catch (const std::exception& e) {                                                        \
+        cmcx::set_error(e.what());                                                             \
+        return CMCX_ERR_ARG;                                                                   \
+    } catch (const std::string& e) {                                                           \
+        cmcx::set_error(e);                                                                    \
+        return CMCX_ERR_ARG;                                                                   \
+    }
+
+#define SOLVER_CALL(s, expr)                  \
+    do {                                      \
+        if (!(s) || !(s)->impl) return CMCX_ERR_ARG; \
+        CMCX_GUARD(return (s)->impl->expr;)   \
+    } while (0)
+
+extern "C" {
+
+const char* cmcx_version(void) { return "cmcx 0.1 (gfx950)"; }
+const char* cmcx_last_error(void) { return cmcx::g_last_error.c_str(); }
+
+void cmcx_params_normalized(double Re, double Pr, double lambda, cmcx_params* out) {
+    if (!out) return;
+    out->v_T = 1.0;
+    out->v_vis = 1.0 / Re;
+    out->t_vis = 1.0 / (Re * Pr);
+    out->t_phi = (lambda - 1) / (lambda * Re);
+}
+
+void cmcx_params_dimensional(double vis, double rho, double R, double k, double cv,
+                             cmcx_params* out) {
+    if (!out) return;
+    out->v_T = R;
+    out->v_vis = vis / rho;
+    out->t_vis = k / (rho * cv);
+    out->t_phi = vis / (rho * cv);
+}
+
+int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, void* stream,
+                cmcx_solver** out) {
+    if (!dom || !params || !out) return CMCX_ERR_ARG;
+    *out = nullptr;
+    if (dtype != CMCX_F32 && dtype != CMCX_F64) return CMCX_ERR_ARG;
+    for (int a = 0; a < 3; a++)
+        if (dom->dims[a] < 3) {
+            set_error("every dimension must be >= 3");
+            return CMCX_ERR_ARG;
+        }
+    if (!dom->type || !dom->bc_vel || !dom->bc_temp || !dom->v[0] || !dom->v[1] || !dom->v[2] ||
+        !dom->T)
+        return CMCX_ERR_ARG;
+    int64_t N = (int64_t)dom->dims[0] * dom->dims[1] * dom->dims[2];
+    if (N > ((int64_t)1 << 31) - 1) {
+        set_error("grid too large (node count must fit in int32)");
+        return CMCX_ERR_TOO_LARGE;
+    }
+    for (int64_t i = 0; i < N; i++)
+        if (dom->type[i] > 3 || dom->bc_vel[i] > 1 || dom->bc_temp[i] > 1) {
+            set_error("node type or BC kind out of range at node " + std::to_string(i));
+            return CMCX_ERR_ARG;
+        }
+    CMCX_GUARD({
+        auto* s = new cmcx_solver;
+        s->dtype = dtype;
+        hipStream_t st = (hipStream_t)stream;
+        if (dtype == CMCX_F32) s->impl.reset(new Solver<float>(dom, params, st));
+        else s->impl.reset(new Solver<double>(dom, params, st));
+        int rc = dtype == CMCX_F32 ? static_cast<Solver<float>*>(s->impl.get())->check_mask()
+                                   : static_cast<Solver<double>*>(s->impl.get())->check_mask();
+        if (rc != CMCX_OK) {
+            delete s;
+            return rc;
+        }
+        *out = s;
+        return CMCX_OK;
+    })
+}
+
+int cmcx_destroy(cmcx_solver* s) {
+    if (!s) return CMCX_ERR_ARG;
+    CMCX_GUARD({
+        delete s;
+        return CMCX_OK;
+    })
+}
+
+int cmcx_set_options(cmcx_solver* s, int path, int merge_order) { SOLVER_CALL(s, set_options(path, merge_order)); }
+int cmcx_set_nblockz(cmcx_solver* s, int nb) { SOLVER_CALL(s, set_nblockz(nb)); }
+int cmcx_build_segments(cmcx_solver* s) { SOLVER_CALL(s, build_segments()); }
+int cmcx_num_segments(cmcx_solver* s, int axis, int64_t* n) { SOLVER_CALL(s, num_segments(axis, n)); }
+int cmcx_get_segments(cmcx_solver* s, int axis, int32_t* out, int64_t cap) { SOLVER_CALL(s, get_segments(axis, out, cap)); }
+int cmcx_get_segment_nodes(cmcx_solver* s, int axis, uint8_t* flags, double* vals, int64_t cap) {
+    SOLVER_CALL(s, get_segment_nodes(axis, flags, vals, cap));
+}
+int cmcx_update_boundaries(cmcx_solver* s) { SOLVER_CALL(s, update_boundaries()); }
+int cmcx_step(cmcx_solver* s, double dt, int G, int L, int compute_err, double* err) {
+    SOLVER_CALL(s, step(dt, G, L, compute_err, err));
+}
+int cmcx_run(cmcx_solver* s, double dt, int steps, int G, int L, double* err) {
+    if (!s || !s->impl) return CMCX_ERR_ARG;
+    CMCX_GUARD({
+        int rc = CMCX_OK;
+        for (int i = 0; i < steps; i++) {
+            rc = s->impl->update_boundaries();
+            if (rc != CMCX_OK) return rc;
+            rc = s->impl->step(dt, G, L, (i % 10 == 0) || (i == steps - 1), err);
+            if (rc != CMCX_OK) return rc;
+        }
+        return rc;
+    })
+}
+int cmcx_div_error(cmcx_solver* s, double* err) { SOLVER_CALL(s, div_error(err)); }
+int cmcx_get_fields(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T) {
+    SOLVER_CALL(s, get_fields(u, v, w, p, T, false));
+}
+int cmcx_get_fields_device(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T) {
+    SOLVER_CALL(s, get_fields(u, v, w, p, T, true));
+}
+int cmcx_set_fields(cmcx_solver* s, const void* u, const void* v, const void* w, const void* T) {
+    SOLVER_CALL(s, set_fields(u, v, w, T));
+}
+int cmcx_state_device_ptrs(cmcx_solver* s, void** p4) { SOLVER_CALL(s, state_ptrs(p4)); }
+int cmcx_get_layer(cmcx_solver* s, void* vel, double* T, int ox, int oy, int oz) {
+    SOLVER_CALL(s, get_layer(vel, T, ox, oy, oz));
+}
+int cmcx_set_boundary(cmcx_solver* s, const uint8_t* bv, const uint8_t* bt, const double* vx,
+                      const double* vy, const double* vz, const double* T) {
+    SOLVER_CALL(s, set_boundary(bv, bt, vx, vy, vz, T));
+}
+int cmcx_profile(cmcx_solver* s, int enable) { SOLVER_CALL(s, profile(enable)); }
+int cmcx_profile_read(cmcx_solver* s, double* ms, int64_t* launches, int reset) {
+    SOLVER_CALL(s, profile_read(ms, launches, reset));
+}
+int cmcx_step_bytes(cmcx_solver* s, int G, int L, double* bytes) { SOLVER_CALL(s, step_bytes(G, L, bytes)); }
+int cmcx_synchronize(cmcx_solver* s) { SOLVER_CALL(s, synchronize()); }
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP translation units of libcmcx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cmcx.h"
+
+namespace cmcx {
+
+enum { NODE_IN = 0, NODE_OUT = 1, NODE_BOUND = 2, NODE_VALVE = 3 };
+enum { BC_NOSLIP = 0, BC_FREE = 1 };
+enum { AX_X = 0, AX_Y = 1, AX_Z = 2 };
+
+void set_error(const std::string& msg);
+
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+
+#define CMCX_HIP(expr)                                                            \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) throw ::cmcx::HipError{_e, std::string(#expr)};     \
+    } while (0)
+
+#define CMCX_LAUNCH_CHECK()                                                       \
+    do {                                                                          \
+        hipError_t _e = hipGetLastError();                                        \
+        if (_e != hipSuccess) throw ::cmcx::HipError{_e, std::string(__func__)};  \
+    } while (0)
+
+// Geometry of one grid (dimensions and strides of the x-major, z-fastest layout).
+struct Geom {
+    int nx, ny, nz;
+    int64_t N;
+    int64_t sx, sy;  // strides of i and j (sz = 1)
+};
+
+// Segment lists of one axis as structure-of-arrays on the device.
+struct SegList {
+    int64_t n = 0;
+    int max_size = 0;
+    int64_t sum_size = 0;      // S_d: sum of segment sizes, endpoints included
+    int32_t* d = nullptr;      // 7 arrays of n int32: px, py, pz, ex, ey, ez, size
+};
+
+// Field-type dependent parameters of one sweep, precomputed on the host in the
+// field type T exactly as the reference computes them (AdiSolver3D.cpp:739-753).
+template <class T>
+struct SweepParams {
+    T dt;
+    T h;          // spacing along the sweep axis
+    T hx, hy, hz; // spacings (stencils of the dissipation terms)
+    T vis_v;      // v_vis / h^2
+    T vis_t;      // t_vis / h^2
+    T v_T, t_phi;
+};
+
+template <class T>
+struct Layer4 {
+    T* f[4];
+};
+
+// Segment builder (kernels_segments.hip)
+void build_segment_list(const uint8_t* d_type, const Geom& g, int axis, int nblockz,
+                        SegList& out, hipStream_t st);
+
+// Plain sweep (kernels_plain.hip)
+template <class T>
+void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const SweepParams<T>& p,
+                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
+                        Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
+                        int var0, int var1, T* scratch, hipStream_t st);
+
+// Fused sweep (kernels_fused.hip)
+template <class T>
+bool fused_supported(const Geom& g);
+template <class T>
+void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
+                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
+                        Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
+                        const int32_t* lines, int64_t nlines, hipStream_t st);
+// Active lines of an axis (lines holding at least one segment), line ids in
+// the natural order of the fused kernels.
+void build_active_lines(const uint8_t* d_type, const Geom& g, int axis, int32_t** d_lines,
+                        int64_t* nlines, hipStream_t st);
+
+// Field kernels (kernels_fields.hip)
+template <class T>
+void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const* gval,
+                              Layer4<T> cur, hipStream_t st);
+template <class T>
+void launch_prep_step(const Geom& g, const uint8_t* type, Layer4<T> cur, Layer4<T> next,
+                      Layer4<T> tA, Layer4<T> tB, hipStream_t st);
+template <class T>
+void launch_div_error(const Geom& g, const uint8_t* type, Layer4<T> L, T hx, T hy, T hz,
+                      double* d_acc, hipStream_t st);
+template <class T>
+void launch_scale(const T* src, T* dst, T a, int64_t n, hipStream_t st);
+template <class T>
+void launch_convert_from_f64(const double* src, T* dst, int64_t n, hipStream_t st);
+template <class T>
+void launch_copy_typed(const Geom& g, const uint8_t* type, int target, Layer4<T> src,
+                       Layer4<T> dst, hipStream_t st);
+template <class T>
+void launch_fill_typed(const Geom& g, const uint8_t* type, int target, Layer4<T> dst, T u, T v,
+                       T w, T t, hipStream_t st);
+void launch_zero_f64(double* p, int n, hipStream_t st);
+// mask checks: out[0] = #IN nodes on box faces, out[1] = #OUT nodes 6-adjacent to IN
+void launch_mask_check(const Geom& g, const uint8_t* type, unsigned long long* d_out,
+                       hipStream_t st);
+
+}  // namespace cmcx

@@ -1,0 +1,202 @@
+/*
+ * cmcx.h -- C ABI of the MI355X-native ADI time step (drop-in for the
+ * AdiSolver3D path of samech-git/cmc-fluid-solver).
+ *
+ * Plain C: opaque handle, pointers and sizes, int status codes, no exceptions
+ * and no torch types across the boundary.  One handle is driven by one host
+ * thread at a time; all device work of a handle is issued on its stream.
+ *
+ * Reference interfaces each entry point replaces (paths are inside the
+ * reference tree):
+ *   cmcx_create           AdiSolver3D::Init                 src/FluidSolver3D/AdiSolver3D.cpp:166-268
+ *                         + Grid3D node data (SetType/SetData) src/FluidSolver3D/Grid3D.h:120-121
+ *   cmcx_set_options      AdiSolver3D::SetOptionsGPU        src/FluidSolver3D/AdiSolver3D.cpp:160-164
+ *   cmcx_build_segments   AdiSolver3D::CreateSegments       src/FluidSolver3D/AdiSolver3D.cpp:553-562
+ *   cmcx_get_segments     h_listX/Y/Z (Segment3D lists)     src/FluidSolver3D/AdiSolver3D.h:78
+ *   cmcx_get_segment_nodes  d_node_listX/Y/Z (NodesBoundary3D) src/FluidSolver3D/Grid3D.cpp:129-146
+ *   cmcx_update_boundaries AdiSolver3D::UpdateBoundaries    src/FluidSolver3D/AdiSolver3D.cpp:286-304
+ *   cmcx_step             AdiSolver3D::TimeStep             src/FluidSolver3D/AdiSolver3D.cpp:306-391
+ *   cmcx_div_error        TimeLayer3D::EvalDivError         src/FluidSolver3D/TimeLayer3D.h:595-641
+ *   cmcx_get_layer        Solver3D::GetLayer                src/FluidSolver3D/Solver3D.cpp:21-25
+ *   cmcx_get_fields       (current state u,v,w,p,T; the reference keeps it in the
+ *                          protected `cur` layer, src/FluidSolver3D/Solver3D.h:46)
+ *   cmcx_set_fields       Solver2D::SetLayer analogue       src/FluidSolver2D/Solver2D.cpp:37-46
+ *   cmcx_set_boundary     Grid3D::SetData                   src/FluidSolver3D/Grid3D.h:121
+ *   cmcx_params_*         FluidParams constructors          src/Common/Geometry.h:545-561
+ *   cmcx_destroy          AdiSolver3D::~AdiSolver3D         src/FluidSolver3D/AdiSolver3D.cpp:153-158
+ *
+ * 2D ADI (the reference's CPU-only FluidSolver2D path):
+ *   cmcx2d_create/update_boundaries/step/get_fields/destroy
+ *                         AdiSolver2D::Init/TimeStep, Solver2D::UpdateBoundaries
+ *                         src/FluidSolver2D/AdiSolver2D.cpp:21-50, 279-323; Solver2D.cpp:48-63
+ */
+#ifndef CMCX_H
+#define CMCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CMCX_OK 0
+#define CMCX_DIVERGED 1          /* divergence guard fired (mean |div| > 0.01); state not advanced */
+#define CMCX_ERR_ARG (-1)        /* bad argument (null pointer, size, enum) */
+#define CMCX_ERR_MASK (-2)       /* mask not well formed: NODE_IN on a box face, or NODE_OUT next to NODE_IN */
+#define CMCX_ERR_HIP (-3)        /* HIP runtime error (message in cmcx_last_error) */
+#define CMCX_ERR_STATE (-4)      /* call out of order (e.g. step before build_segments) */
+#define CMCX_ERR_UNSUPPORTED (-5)
+#define CMCX_ERR_TOO_LARGE (-6)
+#define CMCX_ERR_PICARD (-7)     /* 2D: Picard loop exceeded 100 iterations or error > 1.0 */
+
+/* precision of the fields (the reference's FTYPE) */
+#define CMCX_F32 0
+#define CMCX_F64 1
+
+/* node types and boundary kinds, same numbering as the reference's enums
+   (src/Common/Geometry.h:31-41) */
+#define CMCX_NODE_IN 0
+#define CMCX_NODE_OUT 1
+#define CMCX_NODE_BOUND 2
+#define CMCX_NODE_VALVE 3
+#define CMCX_BC_NOSLIP 0
+#define CMCX_BC_FREE 1
+
+/* axes (DirType X, Y, Z) */
+#define CMCX_X 0
+#define CMCX_Y 1
+#define CMCX_Z 2
+
+/* sweep implementations (cmcx_set_options `path`) */
+#define CMCX_PATH_PLAIN 0   /* one thread per (segment, field), one launch per axis */
+#define CMCX_PATH_FUSED 1   /* one wavefront per grid line, LDS-staged tiles, one launch per axis */
+
+/* Picard merge order (cmcx_set_options `merge_order`) */
+#define CMCX_MERGE_CPU 0        /* the reference's CPU backend (AdiSolver3D.cpp:591-604, 347-357) */
+#define CMCX_MERGE_GPU_ORDER 1  /* the reference's CUDA backend (AdiSolver3D.cu:337-364): per field, in order */
+
+typedef struct cmcx_solver cmcx_solver;
+
+/* Per-node input, x-major with z fastest: node (i,j,k) is element (i*dimy + j)*dimz + k.
+   Structure-of-arrays form of the reference's Node array (Grid3D.h:73-88). */
+typedef struct cmcx_domain {
+    int32_t dims[3];          /* dimx, dimy, dimz (each >= 3) */
+    double h[3];              /* dx, dy, dz */
+    double base_t;            /* Grid3D::baseT */
+    const uint8_t* type;      /* NodeType per node */
+    const uint8_t* bc_vel;    /* BCtype for u, v, w per node */
+    const uint8_t* bc_temp;   /* BCtype for T per node */
+    const double* v[3];       /* node velocity (Node.v), used on BOUND/VALVE nodes */
+    const double* T;          /* node temperature (Node.T) */
+} cmcx_domain;
+
+/* FluidParams (Geometry.h:538-562); rounded to the field type inside. */
+typedef struct cmcx_params {
+    double v_T, v_vis, t_vis, t_phi;
+} cmcx_params;
+
+const char* cmcx_version(void);
+/* message of the last failing call on this host thread */
+const char* cmcx_last_error(void);
+
+void cmcx_params_normalized(double Re, double Pr, double lambda, cmcx_params* out);
+void cmcx_params_dimensional(double vis, double rho, double R, double k, double cv,
+                             cmcx_params* out);
+
+/* Allocates the layers on the current HIP device and uploads the domain.
+   State starts as the node values (TimeLayer3D(grid) -> CopyFromGrid).
+   stream: a hipStream_t, or NULL for the default stream. */
+int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, void* stream,
+                cmcx_solver** out);
+int cmcx_destroy(cmcx_solver* s);
+
+int cmcx_set_options(cmcx_solver* s, int path, int merge_order);
+/* Z-block reordering of the X and Y segment lists (the reference's nblockZ,
+   Grid3D.cpp:59-62, 124-125).  Affects the list order only; 1 = none. */
+int cmcx_set_nblockz(cmcx_solver* s, int nblockz);
+
+/* Builds the X, Y, Z segment lists on the device (count, scan, emit). */
+int cmcx_build_segments(cmcx_solver* s);
+int cmcx_num_segments(cmcx_solver* s, int axis, int64_t* n);
+/* 7 int32 per segment: posx, posy, posz, endx, endy, endz, size; in the
+   reference's list order.  cap = capacity in segments. */
+int cmcx_get_segments(cmcx_solver* s, int axis, int32_t* out, int64_t cap);
+/* Endpoint node records (NodesBoundary3D: first, last).  Per segment:
+   flags[6] = {first.type, first.bc_vel, first.bc_temp, last.type, last.bc_vel, last.bc_temp},
+   vals[8]  = {first.v.x, first.v.y, first.v.z, first.T, last.v.x, last.v.y, last.v.z, last.T}. */
+int cmcx_get_segment_nodes(cmcx_solver* s, int axis, uint8_t* flags, double* vals, int64_t cap);
+
+/* Writes node velocity/temperature into the state on NODE_BOUND and NODE_VALVE nodes. */
+int cmcx_update_boundaries(cmcx_solver* s);
+
+/* One ADI time step: num_global Picard iterations of Z, Y, X sweeps with
+   num_local sweeps each.  compute_err != 0 evaluates the divergence of the new
+   state into *err (may be NULL).  Returns CMCX_DIVERGED (state not advanced)
+   when the error exceeds 0.01, as the reference throws before its swap. */
+int cmcx_step(cmcx_solver* s, double dt, int num_global, int num_local, int compute_err,
+              double* err);
+/* Several steps with update_boundaries before each (the reference's time loop,
+   FluidSolver3D.cpp:226-242): error evaluated every 10th step and on the last. */
+int cmcx_run(cmcx_solver* s, double dt, int steps, int num_global, int num_local, double* err);
+
+/* Divergence error of the current state. */
+int cmcx_div_error(cmcx_solver* s, double* err);
+
+/* Current state to host arrays of the field type (float for CMCX_F32, double for
+   CMCX_F64), N elements each; any pointer may be NULL.  p = v_T * T. */
+int cmcx_get_fields(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T);
+/* Same into device buffers (N elements each), ordered on the handle's stream. */
+int cmcx_get_fields_device(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T);
+/* Replace the current state (initial conditions, checkpoint restore). */
+int cmcx_set_fields(cmcx_solver* s, const void* u, const void* v, const void* w, const void* T);
+/* Device pointers of the current state fields {u, v, w, T} (valid until the next step). */
+int cmcx_state_device_ptrs(cmcx_solver* s, void** ptrs4);
+
+/* Solver3D::GetLayer: the reference output view.  Sets NODE_OUT nodes of the
+   layer it reads to 99999 and resamples it to (ox, oy, oz) by nearest index
+   (0 = full size).  Like the reference, it reads the buffer that holds the
+   state before the last step.  vel: 3 values per output node (field type),
+   T: double per output node. */
+int cmcx_get_layer(cmcx_solver* s, void* vel, double* T, int ox, int oy, int oz);
+
+/* Grid3D::SetData for every node at once (new boundary data; the mask is fixed). */
+int cmcx_set_boundary(cmcx_solver* s, const uint8_t* bc_vel, const uint8_t* bc_temp,
+                      const double* vx, const double* vy, const double* vz, const double* T);
+
+/* Kernel timing with HIP events on the handle's stream (0 = off). */
+int cmcx_profile(cmcx_solver* s, int enable);
+/* Accumulated time of the sweep kernels since the last reset: total ms and launches. */
+int cmcx_profile_read(cmcx_solver* s, double* sweep_ms, int64_t* sweep_launches, int reset);
+/* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
+int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
+int cmcx_synchronize(cmcx_solver* s);
+
+/* ---- 2D ADI (the reference's FluidSolver2D path) -------------------------- */
+typedef struct cmcx2d_solver cmcx2d_solver;
+typedef struct cmcx2d_domain {
+    int32_t dims[2];          /* dimx, dimy; node (i,j) is element i*dimy + j */
+    double h[2];
+    double start_t;           /* Grid2D::startT */
+    const uint8_t* type;      /* NodeType */
+    const uint8_t* bc;        /* CondData2D.type (BCtype) */
+    const double* v[2];
+    const double* T;
+} cmcx2d_domain;
+
+int cmcx2d_create(const cmcx2d_domain* dom, const cmcx_params* params, int dtype, void* stream,
+                  cmcx2d_solver** out);
+int cmcx2d_update_boundaries(cmcx2d_solver* s);
+/* Returns CMCX_ERR_PICARD where the reference exits (AdiSolver2D.cpp:303-313).
+   iters: Picard iterations done (may be NULL). */
+int cmcx2d_step(cmcx2d_solver* s, double dt, int num_global, int num_local, double* err,
+                int* iters);
+int cmcx2d_get_fields(cmcx2d_solver* s, void* u, void* v, void* p, void* T);
+int cmcx2d_destroy(cmcx2d_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMCX_H */

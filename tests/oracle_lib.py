@@ -1,0 +1,234 @@
+"""ctypes access to the oracle (oracle/_build/liboracle.so) and to the
+reference-header pieces (oracle/_ref/libref.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libref.so")
+
+_c_u8 = ctypes.POINTER(ctypes.c_uint8)
+_c_d = ctypes.POINTER(ctypes.c_double)
+_c_f = ctypes.POINTER(ctypes.c_float)
+_c_i32 = ctypes.POINTER(ctypes.c_int32)
+
+_lib = None
+_ref = None
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-C", ORACLE_DIR], stdout=subprocess.DEVNULL)
+    L = ctypes.CDLL(ORACLE_SO)
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             _c_u8, _c_u8, _c_u8, _c_d, _c_d, _c_d, _c_d,
+                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_int, ctypes.c_int]
+    for name in ("orc_destroy", "orc_create_segments", "orc_update_boundaries"):
+        getattr(L, name).argtypes = [ctypes.c_void_p]
+        getattr(L, name).restype = None
+    L.orc_num_segments.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_num_segments.restype = ctypes.c_int
+    L.orc_get_segments.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_i32]
+    L.orc_get_segments.restype = None
+    L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, _c_d]
+    L.orc_step.restype = ctypes.c_int
+    L.orc_get_layer_raw.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_d, _c_d, _c_d, _c_d]
+    L.orc_get_layer_raw.restype = None
+    L.orc_set_fields.argtypes = [ctypes.c_void_p, _c_d, _c_d, _c_d, _c_d]
+    L.orc_set_fields.restype = None
+    L.orc_div_error_cur.argtypes = [ctypes.c_void_p]
+    L.orc_div_error_cur.restype = ctypes.c_double
+    L.orc_get_layer.argtypes = [ctypes.c_void_p, _c_f, _c_d, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_get_layer.restype = None
+    L.orc_thomas_f64.argtypes = [_c_d] * 5 + [ctypes.c_int]
+    L.orc_thomas_f32.argtypes = [_c_f] * 5 + [ctypes.c_int]
+    L.orc_segments_count.argtypes = [_c_u8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_segments_count.restype = ctypes.c_int
+    L.orc_segments.argtypes = [_c_u8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_i32]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_max_threads.restype = ctypes.c_int
+    # 2D
+    L.orc2_create.restype = ctypes.c_void_p
+    L.orc2_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, _c_u8, _c_u8, _c_d, _c_d, _c_d,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.orc2_destroy.argtypes = [ctypes.c_void_p]
+    L.orc2_update_boundaries.argtypes = [ctypes.c_void_p]
+    L.orc2_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_d,
+                            ctypes.POINTER(ctypes.c_int)]
+    L.orc2_step.restype = ctypes.c_int
+    L.orc2_get_layer_raw.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_d, _c_d, _c_d]
+    L.orc2_segments.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_i32]
+    L.orc2_segments.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def load_ref():
+    """The reference's own Thomas / FluidParams compiled from its headers, or None."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    if not os.path.exists(REF_SO):
+        if os.path.isdir("/root/reference/src/Common"):
+            subprocess.call(["make", "-C", ORACLE_DIR, "ref"], stdout=subprocess.DEVNULL)
+        if not os.path.exists(REF_SO):
+            return None
+    R = ctypes.CDLL(REF_SO)
+    R.ref_thomas_f32.argtypes = [_c_f] * 5 + [ctypes.c_int]
+    R.ref_thomas_f64.argtypes = [_c_d] * 5 + [ctypes.c_int]
+    R.ref_fluid_params_normalized.argtypes = [ctypes.c_double] * 3 + [_c_f]
+    R.ref_fluid_params_dimensional.argtypes = [ctypes.c_double] * 5 + [_c_f]
+    _ref = R
+    return R
+
+
+def segments(type_u8: np.ndarray, shape, axis: int, nblockz: int = 1) -> np.ndarray:
+    L = load()
+    t = np.ascontiguousarray(type_u8, dtype=np.uint8)
+    nx, ny, nz = shape
+    n = L.orc_segments_count(_ptr(t, _c_u8), nx, ny, nz, axis, nblockz)
+    out = np.zeros((max(n, 1), 7), np.int32)
+    L.orc_segments(_ptr(t, _c_u8), nx, ny, nz, axis, nblockz, _ptr(out, _c_i32))
+    return out[:n]
+
+
+class Oracle3D:
+    """CPU restatement of AdiSolver3D (CPU backend) for one domain."""
+
+    def __init__(self, dom, dtype: str = "f64", merge_mode: int = 0, params=None, nblockz: int = 1):
+        L = load()
+        self.L = L
+        self.dom = dom
+        self.shape = dom.shape
+        p = params or dom.params()
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in (dom.type, dom.bc_vel, dom.bc_temp)]
+        vals = [np.ascontiguousarray(a, dtype=np.float64) for a in (dom.vx, dom.vy, dom.vz, dom.T)]
+        self._keep = arrs + vals
+        self.h = L.orc_create(0 if dtype == "f32" else 1, dom.nx, dom.ny, dom.nz, dom.dx, dom.dy,
+                              dom.dz, *[_ptr(a, _c_u8) for a in arrs], *[_ptr(a, _c_d) for a in vals],
+                              p.v_T, p.v_vis, p.t_vis, p.t_phi, merge_mode, nblockz)
+        self.p = p
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def create_segments(self):
+        self.L.orc_create_segments(self.h)
+
+    def get_segments(self, axis: int) -> np.ndarray:
+        n = self.L.orc_num_segments(self.h, axis)
+        out = np.zeros((max(n, 1), 7), np.int32)
+        self.L.orc_get_segments(self.h, axis, _ptr(out, _c_i32))
+        return out[:n]
+
+    def update_boundaries(self):
+        self.L.orc_update_boundaries(self.h)
+
+    def step(self, dt, G=4, L=1, compute_err=True):
+        err = ctypes.c_double(0.0)
+        r = self.L.orc_step(self.h, dt, G, L, 1 if compute_err else 0, ctypes.byref(err))
+        return r, err.value
+
+    def layer(self, which: int = 0):
+        n = self.dom.n
+        out = [np.zeros(n) for _ in range(4)]
+        self.L.orc_get_layer_raw(self.h, which, *[_ptr(a, _c_d) for a in out])
+        return out
+
+    def fields(self):
+        return self.layer(0)
+
+    def set_fields(self, u, v, w, T):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, T)]
+        self.L.orc_set_fields(self.h, *[_ptr(a, _c_d) for a in arrs])
+
+    def div_error(self):
+        return self.L.orc_div_error_cur(self.h)
+
+    def get_layer(self, ox=0, oy=0, oz=0):
+        nx, ny, nz = self.shape
+        ox, oy, oz = ox or nx, oy or ny, oz or nz
+        v = np.zeros((ox * oy * oz, 3), np.float32)
+        T = np.zeros(ox * oy * oz)
+        self.L.orc_get_layer(self.h, _ptr(v, _c_f), _ptr(T, _c_d), ox, oy, oz)
+        return v, T
+
+    def run(self, steps, dt=None, G=None, L=None, compute_every=10):
+        d = self.dom
+        dt = d.dt if dt is None else dt
+        G = d.num_global if G is None else G
+        L = d.num_local if L is None else L
+        for i in range(steps):
+            self.update_boundaries()
+            r, err = self.step(dt, G, L, (i % compute_every == 0) or (i == steps - 1))
+            if r != 0:
+                return r, err
+        return 0, err
+
+
+class Oracle2D:
+    def __init__(self, dom, dtype: str = "f64", params=None):
+        L = load()
+        self.L = L
+        self.dom = dom
+        from cmcx.params import FluidParams
+        p = params or FluidParams.normalized(dom.re, 0.72, 1.4)
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in (dom.type, dom.bc)]
+        vals = [np.ascontiguousarray(a, dtype=np.float64) for a in (dom.vx, dom.vy, dom.T)]
+        self._keep = arrs + vals
+        self.h = L.orc2_create(0 if dtype == "f32" else 1, dom.nx, dom.ny, dom.dx, dom.dy,
+                               dom.base_t, *[_ptr(a, _c_u8) for a in arrs],
+                               *[_ptr(a, _c_d) for a in vals], p.v_T, p.v_vis, p.t_vis, p.t_phi)
+
+    def __del__(self):
+        try:
+            self.L.orc2_destroy(self.h)
+        except Exception:
+            pass
+
+    def update_boundaries(self):
+        self.L.orc2_update_boundaries(self.h)
+
+    def step(self, dt, G=4, L=1):
+        err = ctypes.c_double(0.0)
+        it = ctypes.c_int(0)
+        r = self.L.orc2_step(self.h, dt, G, L, ctypes.byref(err), ctypes.byref(it))
+        return r, err.value, it.value
+
+    def layer(self, which=0):
+        n = self.dom.nx * self.dom.ny
+        out = [np.zeros(n) for _ in range(3)]
+        self.L.orc2_get_layer_raw(self.h, which, *[_ptr(a, _c_d) for a in out])
+        return out
+
+    def segments(self, axis):
+        n = self.L.orc2_segments(self.h, axis, ctypes.cast(None, _c_i32))
+        out = np.zeros((max(n, 1), 5), np.int32)
+        self.L.orc2_segments(self.h, axis, _ptr(out, _c_i32))
+        return out[:n]
