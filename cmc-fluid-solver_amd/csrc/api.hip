@@ -70,7 +70,9 @@ class Solver final : public ISolver {
     size_t scratch_elems = 0;
     int32_t* lines[3] = {nullptr, nullptr, nullptr};
     int64_t nlines[3] = {0, 0, 0};
+    int free_shared = 0;  // axes that need the per-segment kernel (shared free nodes)
     double* d_acc = nullptr;
+    double* d_part = nullptr;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -102,6 +104,7 @@ class Solver final : public ISolver {
         for (int l = 0; l < 5; l++)
             for (int f = 0; f < 4; f++) Ls[l]->f[f] = buf + (int64_t)(l * 4 + f) * N;
         CMCX_HIP(hipMalloc((void**)&d_acc, sizeof(double) * 2));
+        CMCX_HIP(hipMalloc((void**)&d_part, sizeof(double) * div_partials()));
         CMCX_HIP(hipMemcpyAsync(d_type, d->type, N, hipMemcpyHostToDevice, st));
         upload_boundary(d->bc_vel, d->bc_temp, d->v[0], d->v[1], d->v[2], d->T);
         // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
@@ -135,6 +138,7 @@ class Solver final : public ISolver {
         hipFree(scratch);
         for (auto* l : lines) hipFree(l);
         hipFree(d_acc);
+        hipFree(d_part);
         for (auto& e : ev_pool) {
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
@@ -183,24 +187,28 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    bool plain_axis(int a) const { return path == CMCX_PATH_PLAIN || ((free_shared >> a) & 1); }
+
     void prepare_path_buffers() {
-        if (path == CMCX_PATH_PLAIN) {
-            size_t need = 0;
-            for (auto& s : segs) need = std::max(need, (size_t)s.max_size * 4 * (size_t)s.n * 2);
-            if (need > scratch_elems) {
-                hipFree(scratch);
-                scratch = nullptr;
-                CMCX_HIP(hipMalloc((void**)&scratch, sizeof(T) * need));
-                scratch_elems = need;
-            }
-        } else {
-            for (int a = 0; a < 3; a++)
-                if (!lines[a]) build_active_lines(d_type, g, a, &lines[a], &nlines[a], st);
+        size_t need = 0;
+        for (int a = 0; a < 3; a++)
+            if (plain_axis(a))
+                need = std::max(need, (size_t)segs[a].max_size * 4 * (size_t)segs[a].n * 2);
+        if (need > scratch_elems) {
+            hipFree(scratch);
+            scratch = nullptr;
+            CMCX_HIP(hipMalloc((void**)&scratch, sizeof(T) * need));
+            scratch_elems = need;
         }
+        if (path == CMCX_PATH_FUSED)
+            for (int a = 0; a < 3; a++)
+                if (!lines[a] && !plain_axis(a))
+                    build_active_lines(d_type, g, a, 128 / (int)sizeof(T), &lines[a], &nlines[a], st);
     }
 
     int build_segments() override {
         for (int a = 0; a < 3; a++) build_segment_list(d_type, g, a, nblockz, segs[a], st);
+        free_shared = shared_free_axes(d_type, d_bcv, d_bct, g, st);
         prepare_path_buffers();
         built = true;
         return CMCX_OK;
@@ -284,6 +292,11 @@ class Solver final : public ISolver {
         p.vis_t = t_vis / (p.h * p.h);
         p.v_T = (T)prm.v_T;
         p.t_phi = (T)prm.t_phi;
+        p.three_dt = (T)3 / dt;
+        p.inv_2h = (T)1 / (2 * p.h);
+        p.inv_2hx = (T)1 / (2 * p.hx);
+        p.inv_2hy = (T)1 / (2 * p.hy);
+        p.inv_2hz = (T)1 / (2 * p.hz);
         return p;
     }
 
@@ -317,8 +330,8 @@ class Solver final : public ISolver {
             ev = next_events();
             CMCX_HIP(hipEventRecord(ev[0], st));
         }
-        if (path == CMCX_PATH_PLAIN)
-            launch_plain_sweep<T>(g, axis, segs[axis], p, d_bcv, d_bct, d_g, in, tin, out, tout,
+        if (plain_axis(axis))
+            launch_plain_sweep<T>(g, axis, segs[axis], p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout,
                                   merges, var0, var1, scratch, st);
         else
             launch_fused_sweep<T>(g, axis, p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout, merges,
@@ -378,8 +391,7 @@ class Solver final : public ISolver {
     }
 
     double eval_div(Layer4<T> Lr) {
-        launch_zero_f64(d_acc, 2, st);
-        launch_div_error<T>(g, d_type, Lr, (T)hx_d, (T)hy_d, (T)hz_d, d_acc, st);
+        launch_div_error<T>(g, d_type, Lr, (T)hx_d, (T)hy_d, (T)hz_d, d_acc, d_part, st);
         double h[2];
         CMCX_HIP(hipMemcpyAsync(h, d_acc, sizeof(h), hipMemcpyDeviceToHost, st));
         CMCX_HIP(hipStreamSynchronize(st));
@@ -464,6 +476,10 @@ class Solver final : public ISolver {
                      const double* vz, const double* Tt) override {
         if (!bv || !bt || !vx || !vy || !vz || !Tt) return CMCX_ERR_ARG;
         upload_boundary(bv, bt, vx, vy, vz, Tt);
+        if (built) {
+            free_shared = shared_free_axes(d_type, d_bcv, d_bct, g, st);
+            prepare_path_buffers();
+        }
         return CMCX_OK;
     }
 

@@ -60,6 +60,10 @@ struct SweepParams {
     T vis_v;      // v_vis / h^2
     T vis_t;      // t_vis / h^2
     T v_T, t_phi;
+    // reciprocals for the fused kernels (multiplications instead of divisions)
+    T three_dt;   // 3 / dt
+    T inv_2h;     // 1 / (2 h)
+    T inv_2hx, inv_2hy, inv_2hz;
 };
 
 template <class T>
@@ -74,7 +78,7 @@ void build_segment_list(const uint8_t* d_type, const Geom& g, int axis, int nblo
 // Plain sweep (kernels_plain.hip)
 template <class T>
 void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const SweepParams<T>& p,
-                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
+                        const uint8_t* type, const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
                         Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
                         int var0, int var1, T* scratch, hipStream_t st);
 
@@ -88,8 +92,13 @@ void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const 
                         const int32_t* lines, int64_t nlines, hipStream_t st);
 // Active lines of an axis (lines holding at least one segment), line ids in
 // the natural order of the fused kernels.
-void build_active_lines(const uint8_t* d_type, const Geom& g, int axis, int32_t** d_lines,
+// lpg: z lines per tile of the strided (X, Y) sweeps.
+void build_active_lines(const uint8_t* d_type, const Geom& g, int axis, int lpg, int32_t** d_lines,
                         int64_t* nlines, hipStream_t st);
+// Axes whose lines hold a BC_FREE node with NODE_IN on both sides (a free node
+// shared by two segments): bit a set for axis a.
+int shared_free_axes(const uint8_t* d_type, const uint8_t* d_bcv, const uint8_t* d_bct,
+                     const Geom& g, hipStream_t st);
 
 // Field kernels (kernels_fields.hip)
 template <class T>
@@ -98,9 +107,11 @@ void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const
 template <class T>
 void launch_prep_step(const Geom& g, const uint8_t* type, Layer4<T> cur, Layer4<T> next,
                       Layer4<T> tA, Layer4<T> tB, hipStream_t st);
+// Number of doubles of the per-block partials buffer of launch_div_error.
+int div_partials();
 template <class T>
 void launch_div_error(const Geom& g, const uint8_t* type, Layer4<T> L, T hx, T hy, T hz,
-                      double* d_acc, hipStream_t st);
+                      double* d_acc, double* d_part, hipStream_t st);
 template <class T>
 void launch_scale(const T* src, T* dst, T a, int64_t n, hipStream_t st);
 template <class T>

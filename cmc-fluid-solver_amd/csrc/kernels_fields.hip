@@ -4,6 +4,8 @@
 
 #include "cmcx_internal.h"
 
+#include <algorithm>
+
 namespace cmcx {
 
 namespace {
@@ -59,18 +61,24 @@ __global__ void prep_step_kernel(int64_t N, const uint8_t* __restrict__ type, La
 
 // EvalDivError (TimeLayer3D.h:595-641): per NODE_IN node with i < nx-1, j < ny-1,
 // k < nz-1, |flux divergence| of the dual cell; field-type sums, double result.
+// Grid-stride loop, one (sum, count) partial per block, summed in block order by
+// div_finish_kernel: the result does not depend on scheduling.
+constexpr int kDivBlocks = 2048;
+
 template <class T>
-__global__ void div_error_kernel(Geom g, const uint8_t* __restrict__ type, const T* __restrict__ U,
-                                 const T* __restrict__ V, const T* __restrict__ W, T hx, T hy,
-                                 T hz, double* __restrict__ acc) {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) div_error_kernel(Geom g, const uint8_t* __restrict__ type,
+                                                        const T* __restrict__ U,
+                                                        const T* __restrict__ V,
+                                                        const T* __restrict__ W, T hx, T hy, T hz,
+                                                        double* __restrict__ part) {
     double val = 0.0, cnt = 0.0;
-    if (id < g.N) {
+    const int64_t sx = g.sx, sy = g.sy;
+    for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < g.N;
+         id += (int64_t)gridDim.x * blockDim.x) {
         int k = (int)(id % g.nz);
         int j = (int)((id / g.nz) % g.ny);
         int i = (int)(id / g.sx);
         if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1 && type[id] == NODE_IN) {
-            const int64_t sx = g.sx, sy = g.sy;
             const int64_t c = id;
             T su = U[c] + U[c - sy] + U[c - sy - 1] + U[c - 1] - U[c - sx] - U[c - sx - sy] -
                    U[c - sx - sy - 1] - U[c - sx - 1];
@@ -81,19 +89,36 @@ __global__ void div_error_kernel(Geom g, const uint8_t* __restrict__ type, const
             double ex = (double)(su * hz * hy) / 4.0;
             double ey = (double)(sv * hx * hz) / 4.0;
             double ez = (double)(sw * hx * hy) / 4.0;
-            val = fabs(ex + ey + ez);
-            cnt = 1.0;
+            val += fabs(ex + ey + ez);
+            cnt += 1.0;
         }
     }
-    // wave reduction, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-        val += __shfl_down(val, off, 64);
-        cnt += __shfl_down(cnt, off, 64);
+    __shared__ double sv_[256], sc_[256];
+    sv_[threadIdx.x] = val;
+    sc_[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sv_[threadIdx.x] += sv_[threadIdx.x + o];
+            sc_[threadIdx.x] += sc_[threadIdx.x + o];
+        }
+        __syncthreads();
     }
-    if ((threadIdx.x & 63) == 0 && cnt > 0.0) {
-        atomicAdd(&acc[0], val);
-        atomicAdd(&acc[1], cnt);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sv_[0];
+        part[2 * blockIdx.x + 1] = sc_[0];
     }
+}
+
+__global__ void div_finish_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nb; b++) {
+        s += part[2 * b];
+        c += part[2 * b + 1];
+    }
+    out[0] = s;
+    out[1] = c;
 }
 
 template <class T>
@@ -170,11 +195,16 @@ void launch_prep_step(const Geom& g, const uint8_t* type, Layer4<T> cur, Layer4<
     CMCX_LAUNCH_CHECK();
 }
 
+int div_partials() { return 2 * kDivBlocks; }
+
 template <class T>
 void launch_div_error(const Geom& g, const uint8_t* type, Layer4<T> L, T hx, T hy, T hz,
-                      double* d_acc, hipStream_t st) {
-    hipLaunchKernelGGL((div_error_kernel<T>), dim3(blocks_for(g.N)), dim3(kBS), 0, st, g, type,
-                       L.f[0], L.f[1], L.f[2], hx, hy, hz, d_acc);
+                      double* d_acc, double* d_part, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(kDivBlocks, (g.N + kBS - 1) / kBS);
+    hipLaunchKernelGGL((div_error_kernel<T>), dim3(nb), dim3(kBS), 0, st, g, type, L.f[0], L.f[1],
+                       L.f[2], hx, hy, hz, d_part);
+    CMCX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(div_finish_kernel, dim3(1), dim3(64), 0, st, d_part, nb, d_acc);
     CMCX_LAUNCH_CHECK();
 }
 
@@ -223,7 +253,7 @@ void launch_mask_check(const Geom& g, const uint8_t* type, unsigned long long* d
     template void launch_prep_step<T>(const Geom&, const uint8_t*, Layer4<T>, Layer4<T>,           \
                                       Layer4<T>, Layer4<T>, hipStream_t);                          \
     template void launch_div_error<T>(const Geom&, const uint8_t*, Layer4<T>, T, T, T, double*,    \
-                                      hipStream_t);                                                \
+                                      double*, hipStream_t);                                       \
     template void launch_scale<T>(const T*, T*, T, int64_t, hipStream_t);                          \
     template void launch_convert_from_f64<T>(const double*, T*, int64_t, hipStream_t);             \
     template void launch_copy_typed<T>(const Geom&, const uint8_t*, int, Layer4<T>, Layer4<T>,     \

@@ -7,7 +7,10 @@
 // laid out (row, field, segment) so that consecutive threads touch consecutive
 // words, then back-substitutes, writing every node of the segment into `out`
 // (UpdateSegment, :707-730) and the Picard merge temp_out = (temp_in + x)/2 on
-// the segment's NODE_IN nodes (TimeLayer3D.h:415-436).
+// the segment's NODE_IN nodes (TimeLayer3D.h:415-436).  A node that ends one
+// segment and starts the next on the same line (a one-node wall) is written by
+// the later segment only, so the result does not depend on thread timing (the
+// reference's threads race there when the wall is BC_FREE).
 //
 // Arithmetic follows the reference's operation order and is compiled without
 // contraction; with IEEE division it reproduces the reference's CPU arithmetic.
@@ -65,7 +68,7 @@ __device__ __forceinline__ T row_rhs(int axis, int var, const T* __restrict__ cu
 template <class T>
 __global__ void __launch_bounds__(256) plain_sweep_kernel(
     Geom g, int axis, const int32_t* __restrict__ segs, int64_t nseg, SweepParams<T> p,
-    const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
+    const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
     const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gt,
     Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges, int var0,
     T* __restrict__ sc, T* __restrict__ sd) {
@@ -117,7 +120,10 @@ __global__ void __launch_bounds__(256) plain_sweep_kernel(
     T* __restrict__ o = out.f[var];
     const T* ti = tin.f[var];
     T* to = tout.f[var];
-    o[c1] = x;
+    const int epos = axis == AX_X ? ex : axis == AX_Y ? ey : ez;
+    const int dim = axis == AX_X ? g.nx : axis == AX_Y ? g.ny : g.nz;
+    const bool shared_end = epos + 1 < dim && type[c1 + stride] == NODE_IN;
+    if (!shared_end) o[c1] = x;
     cell = c1;
     for (int r = n - 2; r >= 1; r--) {
         cell -= stride;
@@ -135,27 +141,30 @@ __global__ void __launch_bounds__(256) plain_sweep_kernel(
 
 template <class T>
 void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const SweepParams<T>& p,
-                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
-                        Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-                        int var0, int var1, T* scratch, hipStream_t st) {
+                        const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
+                        const T* const* gval, Layer4<T> in, Layer4<T> tin, Layer4<T> out,
+                        Layer4<T> tout, int merges, int var0, int var1, T* scratch,
+                        hipStream_t st) {
     if (segs.n == 0) return;
     const int bs = 256;
     dim3 grid((unsigned)((segs.n + bs - 1) / bs), (unsigned)(var1 - var0));
     // scratch: c' block then d' block, each max_size * 4 * nseg
     int64_t half = (int64_t)segs.max_size * 4 * segs.n;
     hipLaunchKernelGGL((plain_sweep_kernel<T>), grid, dim3(bs), 0, st, g, axis, segs.d, segs.n, p,
-                       bcv, bct, gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges,
+                       type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges,
                        var0, scratch, scratch + half);
     CMCX_LAUNCH_CHECK();
 }
 
 template void launch_plain_sweep<float>(const Geom&, int, const SegList&, const SweepParams<float>&,
-                                        const uint8_t*, const uint8_t*, const float* const*,
+                                        const uint8_t*, const uint8_t*, const uint8_t*,
+                                        const float* const*,
                                         Layer4<float>, Layer4<float>, Layer4<float>, Layer4<float>,
                                         int, int, int, float*, hipStream_t);
 template void launch_plain_sweep<double>(const Geom&, int, const SegList&,
                                          const SweepParams<double>&, const uint8_t*,
-                                         const uint8_t*, const double* const*, Layer4<double>,
+                                         const uint8_t*, const uint8_t*, const double* const*,
+                                         Layer4<double>,
                                          Layer4<double>, Layer4<double>, Layer4<double>, int, int,
                                          int, double*, hipStream_t);
 

@@ -17,9 +17,24 @@ pytestmark = pytest.mark.gpu
 PLAIN, FUSED = 0, 1
 
 
+def _free_wall():
+    """Box with a one-node interior wall with free BCs: two X segments share it."""
+    d = domains.random_domain((20, 16, 18), np.random.default_rng(11), 0.0)
+    t = d.type.reshape(d.shape)
+    m = np.zeros(d.shape, bool)
+    m[10, 3:13, 3:15] = True
+    t[m] = domains.NODE_BOUND
+    d.bc_vel.reshape(d.shape)[m] = domains.BC_FREE
+    d.bc_temp.reshape(d.shape)[m] = domains.BC_FREE
+    for a in (d.vx, d.vy, d.vz):
+        a.reshape(d.shape)[m] = 0.0
+    return d
+
+
 def _cases():
     rng = np.random.default_rng(7)
     return {
+        "free_wall_20x16x18": _free_wall(),
         "lid_sphere_24": domains.lid_sphere(24, radius=5.0, dt=1e-3),
         "random_18x20x22": domains.random_domain((18, 20, 22), rng, 0.12),
         "random_free_17x13x21": domains.random_domain((17, 13, 21), rng, 0.10, free_faces=True),
@@ -84,7 +99,7 @@ def _run_pair(d, dtype, path, steps, G=2, L=1, merge_order=0, oracle_dtype=None)
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("path", [PLAIN])
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
 def test_step_fp64(name, path):
     d = CASES[name]
     say(f"[fp64] {name} path={path}")
@@ -103,7 +118,7 @@ def test_step_fp64(name, path):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("path", [PLAIN])
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
 def test_step_fp32(name, path):
     d = CASES[name]
     say(f"[fp32] {name} path={path}")
@@ -138,10 +153,11 @@ def test_gpu_merge_order():
     s.close()
 
 
-def test_local_iterations():
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
+def test_local_iterations(path):
     d = CASES["random_18x20x22"]
-    say("[num_local=2]")
-    s, ref, _ = _run_pair(d, "f64", PLAIN, steps=2, G=2, L=2)
+    say(f"[num_local=2] path={path}")
+    s, ref, _ = _run_pair(d, "f64", path, steps=2, G=2, L=2)
     got = s.fields()
     want = ref.fields()
     vs = vel_norm(*want[:3])
@@ -150,10 +166,11 @@ def test_local_iterations():
     s.close()
 
 
-def test_get_layer_matches_reference_view():
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
+def test_get_layer_matches_reference_view(path):
     d = CASES["lid_sphere_24"]
-    say("[get_layer]")
-    s, ref, _ = _run_pair(d, "f64", PLAIN, steps=2)
+    say(f"[get_layer] path={path}")
+    s, ref, _ = _run_pair(d, "f64", path, steps=2)
     v, T = s.GetLayer(12, 12, 12)
     rv, rT = ref.get_layer(12, 12, 12)
     assert np.allclose(v, rv, rtol=1e-6, atol=1e-7)
@@ -188,3 +205,35 @@ def test_mask_rejected():
     say("[mask check]")
     with pytest.raises(CmcxError):
         make_solver(d, "f32")
+
+
+@pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
+def test_fused_line_lengths(n):
+    """Fused kernels at every line length of the benchmark workloads (chunk and
+    tile padding), FP64 against the FP64 oracle, FP32 against the FP64 oracle."""
+    d = domains.lid_sphere(n, radius=n / 6.0, dt=1e-3)
+    say(f"[fused length] {n}^3")
+    for dtype, tol in (("f64", 1e-11), ("f32", 1e-5)):
+        s, ref, _ = _run_pair(d, dtype, FUSED, steps=2, oracle_dtype="f64")
+        got = s.fields()
+        want = ref.fields()
+        vs = vel_norm(*want[:3])
+        for k in range(3):
+            e = rel_l2_scaled(got[k], want[k], vs)
+            say(f"  {dtype} field {k}: {e:.3e}")
+            assert e < tol
+        assert rel_l2(got[4], want[3]) < tol
+        s.close()
+
+
+def test_fused_anisotropic_grid():
+    d = domains.random_domain((40, 24, 70), np.random.default_rng(5), 0.1, free_faces=True)
+    say("[fused anisotropic] 40x24x70")
+    s, ref, _ = _run_pair(d, "f64", FUSED, steps=2)
+    got = s.fields()
+    want = ref.fields()
+    vs = vel_norm(*want[:3])
+    for k in range(3):
+        assert rel_l2_scaled(got[k], want[k], vs) < 1e-11
+    assert rel_l2(got[4], want[3]) < 1e-11
+    s.close()
