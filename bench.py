@@ -8,6 +8,9 @@ state resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|plain]
 
+The timed region is bracketed by device synchronisations of the HIP runtime the
+solver runs on (torch is not imported: it bundles a second HIP runtime).
+
 Prints one JSON line: cell-updates/s of the headline path, the roofline of the
 dominant kernel (the sweep: algorithmic bytes (16w+1)*S_d per sweep over the
 measured sweep-kernel time from HIP events on the solver's stream), the plain
@@ -30,21 +33,65 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "ADI cell-updates/s + %HBM peak, batched vs plain on 1 GPU; adjoint/forward time ratio"
 
 
+def _hip():
+    """The HIP runtime libcmcx.so is linked against (same process-wide instance)."""
+    import ctypes
+
+    from cmcx._lib import lib
+
+    lib()
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipDeviceSynchronize.restype = ctypes.c_int
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    return h
+
+
 def _sync():
-    try:
-        import torch
-
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except ImportError:
-        pass
+    # device-wide barrier on the runtime the solver uses (one GPU, one process)
+    rc = _hip().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
-def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32"):
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of `mib` MiB (3 times), the reference kernel
+    for scaling the PMC byte counters."""
+    import ctypes
+
+    h = _hip()
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(a), n) == 0 and h.hipMalloc(ctypes.byref(b), n) == 0
+    assert h.hipMemset(a, 1, n) == 0
+    for _ in range(3):
+        assert h.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    _sync()
+    h.hipFree(a)
+    h.hipFree(b)
+
+
+def run_plain_steps(dom, path: int, steps: int, warmup: int, dtype: str = "f32"):
+    """Steps only, no HIP events and no autotuning (for PMC counter runs, where the
+    profiler does not support event timing)."""
+    from cmcx.solver import make_solver
+
+    s = make_solver(dom, dtype, path=path)
+    for i in range(warmup + steps):
+        s.UpdateBoundaries()
+        s.TimeStep(dom.dt, dom.num_global, dom.num_local, False)
+    s.synchronize()
+    s.close()
+
+
+def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32", tune: bool = True):
     from cmcx.solver import make_solver
 
     s = make_solver(dom, dtype, path=path)
     G, L = dom.num_global, dom.num_local
+    variants = s.autotune(3) if (tune and path == 1) else None
     for _ in range(warmup):
         s.UpdateBoundaries()
         s.TimeStep(dom.dt, G, L, False)
@@ -79,6 +126,7 @@ def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32"):
         "sweep_bytes_per_step": step_bytes,
         "sweep_achieved_gbs": step_bytes * steps / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0,
         "div_error": err,
+        "variants": variants,
     }
 
 
@@ -120,6 +168,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--pmc-mode", action="store_true",
+                    help="only run the steps (no events, no autotuning, no JSON line)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (reference kernel for "
+                         "PMC byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -132,6 +185,8 @@ def main():
     from cmcx._lib import CMCX_PATH_FUSED, CMCX_PATH_PLAIN, lib
 
     lib()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     dom = domains.ellipsoids(args.size)
     fused_ok = True
     try:
@@ -142,6 +197,11 @@ def main():
     except Exception:
         fused_ok = False
     head_path = args.path or ("fused" if fused_ok else "plain")
+    if args.pmc_mode:
+        run_plain_steps(dom, CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN,
+                        args.steps, args.warmup)
+        print(f"pmc-mode: {args.steps} steps of the {head_path} path done", flush=True)
+        return
     head = time_path(dom, CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN,
                      args.steps, args.warmup)
     plain = None
@@ -174,6 +234,7 @@ def main():
             "num_global": dom.num_global,
             "num_local": dom.num_local,
             "path": head_path,
+            "tile_variants": head["variants"],
             "parallelism": "single-gpu",
         },
         "roofline": {

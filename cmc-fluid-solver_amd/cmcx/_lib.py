@@ -75,6 +75,8 @@ SIGNATURES = [
     ("cmcx_profile", ctypes.c_int, [_vp, ctypes.c_int]),
     ("cmcx_profile_read", ctypes.c_int, [_vp, _dp, _i64p, ctypes.c_int]),
     ("cmcx_step_bytes", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    ("cmcx_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    ("cmcx_set_tile_variant", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     ("cmcx_synchronize", ctypes.c_int, [_vp]),
     ("cmcx2d_create", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params), ctypes.c_int,
                                      _vp, ctypes.POINTER(_vp)]),

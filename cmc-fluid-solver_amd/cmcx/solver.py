@@ -189,6 +189,15 @@ class AdiSolver3D:
         check(lib().cmcx_step_bytes(self.h, num_global, num_local, ctypes.byref(b)), "cmcx_step_bytes")
         return b.value
 
+    def autotune(self, reps: int = 3):
+        """Pick the fastest strided-tile variant per axis on this domain."""
+        ch = (ctypes.c_int * 3)()
+        check(lib().cmcx_autotune(self.h, reps, ch), "cmcx_autotune")
+        return [ch[i] for i in range(3)]
+
+    def set_tile_variant(self, axis: int, variant: int):
+        check(lib().cmcx_set_tile_variant(self.h, axis, variant), "cmcx_set_tile_variant")
+
     def synchronize(self):
         check(lib().cmcx_synchronize(self.h), "cmcx_synchronize")
 
@@ -206,10 +215,14 @@ class AdiSolver3D:
 
 def make_solver(dom, dtype: str = "f32", path: int = _lib.CMCX_PATH_PLAIN,
                 merge_order: int = _lib.CMCX_MERGE_CPU, params: FluidParams | None = None,
-                nblockz: int = 1) -> AdiSolver3D:
-    """Init + SetOptions + CreateSegments for a domain in one call."""
+                nblockz: int = 1, variant: int | None = None) -> AdiSolver3D:
+    """Init + SetOptions + CreateSegments for a domain in one call; `variant`
+    forces the fused path's X/Y tile variant."""
     s = AdiSolver3D(dtype)
     s.Init(GPU, False, dom, params or dom.params(), nblockZ=nblockz)
     s.set_options(path, merge_order)
     s.CreateSegments()
+    if variant is not None and path == _lib.CMCX_PATH_FUSED:
+        for axis in range(3):
+            s.set_tile_variant(axis, variant)
     return s

@@ -15,7 +15,9 @@
 #include "cmcx_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -24,6 +26,12 @@ namespace cmcx {
 
 thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Autotuned strided-tile variants per (grid shape, field size), process-wide.
+std::map<std::string, std::array<int, 3>>& tune_cache() {
+    static std::map<std::string, std::array<int, 3>> c;
+    return c;
+}
 
 struct ISolver {
     virtual ~ISolver() {}
@@ -45,6 +53,8 @@ struct ISolver {
     virtual int profile(int enable) = 0;
     virtual int profile_read(double* ms, int64_t* launches, int reset) = 0;
     virtual int step_bytes(int G, int L, double* bytes) = 0;
+    virtual int autotune(int reps, int* choice) = 0;
+    virtual int set_tile_variant(int axis, int variant) = 0;
     virtual int synchronize() = 0;
 };
 
@@ -68,11 +78,16 @@ class Solver final : public ISolver {
     SegList segs[3];
     T* scratch = nullptr;
     size_t scratch_elems = 0;
-    int32_t* lines[3] = {nullptr, nullptr, nullptr};
-    int64_t nlines[3] = {0, 0, 0};
+    int32_t* lines[3][kFusedVariants] = {};
+    int64_t nlines[3][kFusedVariants] = {};
+    int variant[3] = {0, 0, 0};  // strided tile width per axis (autotuned)
     int free_shared = 0;  // axes that need the per-segment kernel (shared free nodes)
     double* d_acc = nullptr;
     double* d_part = nullptr;
+    // node lists (fixed mask): non-IN nodes, BOUND/VALVE nodes
+    uint32_t* d_nonin = nullptr;
+    uint32_t* d_bvlist = nullptr;
+    int64_t n_nonin = 0, n_bv = 0;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -106,6 +121,23 @@ class Solver final : public ISolver {
         CMCX_HIP(hipMalloc((void**)&d_acc, sizeof(double) * 2));
         CMCX_HIP(hipMalloc((void**)&d_part, sizeof(double) * div_partials()));
         CMCX_HIP(hipMemcpyAsync(d_type, d->type, N, hipMemcpyHostToDevice, st));
+        {
+            std::vector<uint32_t> nonin, bvl;
+            for (int64_t i = 0; i < N; i++) {
+                const uint8_t t = d->type[i];
+                if (t != NODE_IN) nonin.push_back((uint32_t)i);
+                if (t == NODE_BOUND || t == NODE_VALVE) bvl.push_back((uint32_t)i);
+            }
+            n_nonin = (int64_t)nonin.size();
+            n_bv = (int64_t)bvl.size();
+            CMCX_HIP(hipMalloc((void**)&d_nonin, sizeof(uint32_t) * std::max<int64_t>(n_nonin, 1)));
+            CMCX_HIP(hipMalloc((void**)&d_bvlist, sizeof(uint32_t) * std::max<int64_t>(n_bv, 1)));
+            if (n_nonin)
+                CMCX_HIP(hipMemcpy(d_nonin, nonin.data(), sizeof(uint32_t) * n_nonin,
+                                   hipMemcpyHostToDevice));
+            if (n_bv)
+                CMCX_HIP(hipMemcpy(d_bvlist, bvl.data(), sizeof(uint32_t) * n_bv, hipMemcpyHostToDevice));
+        }
         upload_boundary(d->bc_vel, d->bc_temp, d->v[0], d->v[1], d->v[2], d->T);
         // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
         for (int f = 0; f < 4; f++)
@@ -136,9 +168,12 @@ class Solver final : public ISolver {
         hipFree(buf);
         for (auto& s : segs) hipFree(s.d);
         hipFree(scratch);
-        for (auto* l : lines) hipFree(l);
+        for (auto& la : lines)
+            for (auto* l : la) hipFree(l);
         hipFree(d_acc);
         hipFree(d_part);
+        hipFree(d_nonin);
+        hipFree(d_bvlist);
         for (auto& e : ev_pool) {
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
@@ -202,13 +237,21 @@ class Solver final : public ISolver {
         }
         if (path == CMCX_PATH_FUSED)
             for (int a = 0; a < 3; a++)
-                if (!lines[a] && !plain_axis(a))
-                    build_active_lines(d_type, g, a, 128 / (int)sizeof(T), &lines[a], &nlines[a], st);
+                if (!plain_axis(a)) ensure_lines(a, a == AX_Z ? 0 : variant[a]);
+    }
+
+    void ensure_lines(int a, int v) {
+        if (lines[a][v] || nlines[a][v]) return;
+        build_active_lines(d_type, g, a, strided_lpg<T>(v), &lines[a][v], &nlines[a][v], st);
+        if (nlines[a][v] == 0) nlines[a][v] = -1;  // built, empty
     }
 
     int build_segments() override {
         for (int a = 0; a < 3; a++) build_segment_list(d_type, g, a, nblockz, segs[a], st);
         free_shared = shared_free_axes(d_type, d_bcv, d_bct, g, st);
+        auto it = tune_cache().find(tune_key());
+        if (it != tune_cache().end())
+            for (int a = 0; a < 3; a++) variant[a] = it->second[a];
         prepare_path_buffers();
         built = true;
         return CMCX_OK;
@@ -276,7 +319,7 @@ class Solver final : public ISolver {
     }
 
     int update_boundaries() override {
-        launch_update_boundaries<T>(g, d_type, d_g, cur, st);
+        launch_update_boundaries_list<T>(n_bv, d_bvlist, d_g, cur, st);
         return CMCX_OK;
     }
 
@@ -334,8 +377,12 @@ class Solver final : public ISolver {
             launch_plain_sweep<T>(g, axis, segs[axis], p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout,
                                   merges, var0, var1, scratch, st);
         else
+        {
+            const int v = variant[axis];
+            const int lv = axis == AX_Z ? 0 : v;
             launch_fused_sweep<T>(g, axis, p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout, merges,
-                                  lines[axis], nlines[axis], st);
+                                  lines[axis][lv], std::max<int64_t>(nlines[axis][lv], 0), v, st);
+        }
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
 
@@ -348,7 +395,7 @@ class Solver final : public ISolver {
         const T dt = (T)dt_d;
         SweepParams<T> P[3] = {sweep_params(AX_X, dt), sweep_params(AX_Y, dt), sweep_params(AX_Z, dt)};
         if (merge_order == CMCX_MERGE_CPU) {
-            launch_prep_step<T>(g, d_type, cur, next, tA, tB, st);
+            launch_prep_step_list<T>(n_nonin, d_nonin, d_type, cur, next, tA, tB, st);
             Layer4<T> tin = cur;  // temp == cur at the first sweep
             auto other = [&](const Layer4<T>& t) { return t.f[0] == tB.f[0] ? tA : tB; };
             for (int it = 0; it < G; it++) {
@@ -498,6 +545,64 @@ class Solver final : public ISolver {
             prof_launches = 0;
         }
         return CMCX_OK;
+    }
+
+    // Times every strided-tile variant of the X and Y sweeps on this domain
+    // (scratch layers only: the state is untouched) and keeps the fastest.
+    int autotune(int reps, int* choice) override {
+        if (!built) return CMCX_ERR_STATE;
+        if (reps < 1) reps = 3;
+        if (path == CMCX_PATH_FUSED) {
+            const T dt = (T)1e-4;
+            hipEvent_t e0, e1;
+            CMCX_HIP(hipEventCreate(&e0));
+            CMCX_HIP(hipEventCreate(&e1));
+            for (int a = 0; a < 3; a++) {
+                if (plain_axis(a)) continue;
+                SweepParams<T> P = sweep_params(a, dt);
+                float best = 1e30f;
+                int bestv = variant[a];
+                for (int v = 0; v < kFusedVariants; v++) {
+                    const int lv = a == AX_Z ? 0 : v;  // the Z work list does not depend on v
+                    ensure_lines(a, lv);
+                    auto launch = [&]() {
+                        launch_fused_sweep<T>(g, a, P, d_type, d_bcv, d_bct, d_g, cur, cur, half, tA,
+                                              1, lines[a][lv], std::max<int64_t>(nlines[a][lv], 0),
+                                              v, st);
+                    };
+                    launch();  // warm-up
+                    CMCX_HIP(hipEventRecord(e0, st));
+                    for (int r = 0; r < reps; r++) launch();
+                    CMCX_HIP(hipEventRecord(e1, st));
+                    CMCX_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    CMCX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (ms < best) {
+                        best = ms;
+                        bestv = v;
+                    }
+                }
+                variant[a] = bestv;
+            }
+            CMCX_HIP(hipEventDestroy(e0));
+            CMCX_HIP(hipEventDestroy(e1));
+            tune_cache()[tune_key()] = {variant[0], variant[1], variant[2]};
+        }
+        if (choice)
+            for (int a = 0; a < 3; a++) choice[a] = variant[a];
+        return CMCX_OK;
+    }
+
+    int set_tile_variant(int axis, int v) override {
+        if (axis < 0 || axis > 2 || v < 0 || v >= kFusedVariants) return CMCX_ERR_ARG;
+        variant[axis] = v;
+        if (built) prepare_path_buffers();
+        return CMCX_OK;
+    }
+
+    std::string tune_key() const {
+        return std::to_string(g.nx) + "x" + std::to_string(g.ny) + "x" + std::to_string(g.nz) + "/" +
+               std::to_string(sizeof(T));
     }
 
     int step_bytes(int G, int L, double* bytes) override {
@@ -661,6 +766,10 @@ int cmcx_profile_read(cmcx_solver* s, double* ms, int64_t* launches, int reset) 
     SOLVER_CALL(s, profile_read(ms, launches, reset));
 }
 int cmcx_step_bytes(cmcx_solver* s, int G, int L, double* bytes) { SOLVER_CALL(s, step_bytes(G, L, bytes)); }
+int cmcx_autotune(cmcx_solver* s, int reps, int* choice) { SOLVER_CALL(s, autotune(reps, choice)); }
+int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant) {
+    SOLVER_CALL(s, set_tile_variant(axis, variant));
+}
 int cmcx_synchronize(cmcx_solver* s) { SOLVER_CALL(s, synchronize()); }
 
 }  // extern "C"

@@ -82,14 +82,20 @@ void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const Swee
                         Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
                         int var0, int var1, T* scratch, hipStream_t st);
 
-// Fused sweep (kernels_fused.hip)
+// Fused sweep (kernels_fused.hip).  `variant` selects, for the strided (X, Y)
+// sweeps, the tile width (0 = 128-byte rows, 1 = 64-byte rows; strided_lpg gives
+// its z lines) and, for the Z sweep, the register cap (0 = compiler's choice,
+// 1 = capped for more waves per SIMD).
+constexpr int kFusedVariants = 2;
 template <class T>
 bool fused_supported(const Geom& g);
+template <class T>
+int strided_lpg(int variant);
 template <class T>
 void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                         const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
                         Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-                        const int32_t* lines, int64_t nlines, hipStream_t st);
+                        const int32_t* lines, int64_t nlines, int variant, hipStream_t st);
 // Active lines of an axis (lines holding at least one segment), line ids in
 // the natural order of the fused kernels.
 // lpg: z lines per tile of the strided (X, Y) sweeps.
@@ -104,6 +110,12 @@ int shared_free_axes(const uint8_t* d_type, const uint8_t* d_bcv, const uint8_t*
 template <class T>
 void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const* gval,
                               Layer4<T> cur, hipStream_t st);
+template <class T>
+void launch_update_boundaries_list(int64_t n, const uint32_t* ids, const T* const* gval,
+                                   Layer4<T> cur, hipStream_t st);
+template <class T>
+void launch_prep_step_list(int64_t n, const uint32_t* ids, const uint8_t* type, Layer4<T> cur,
+                           Layer4<T> next, Layer4<T> tA, Layer4<T> tB, hipStream_t st);
 template <class T>
 void launch_prep_step(const Geom& g, const uint8_t* type, Layer4<T> cur, Layer4<T> next,
                       Layer4<T> tA, Layer4<T> tB, hipStream_t st);

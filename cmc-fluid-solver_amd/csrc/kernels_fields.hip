@@ -59,6 +59,41 @@ __global__ void prep_step_kernel(int64_t N, const uint8_t* __restrict__ type, La
     }
 }
 
+// List forms of the two kernels above, over precomputed node lists (the mask is
+// fixed): non-IN nodes for the step set-up, BOUND/VALVE nodes for the boundary
+// fill.  They touch only the listed nodes instead of streaming the whole grid.
+template <class T>
+__global__ void update_boundaries_list_kernel(int64_t n, const uint32_t* __restrict__ ids,
+                                              const T* __restrict__ g0, const T* __restrict__ g1,
+                                              const T* __restrict__ g2, const T* __restrict__ g3,
+                                              Layer4<T> cur) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = ids[t];
+    cur.f[0][i] = g0[i];
+    cur.f[1][i] = g1[i];
+    cur.f[2][i] = g2[i];
+    cur.f[3][i] = g3[i];
+}
+
+template <class T>
+__global__ void prep_step_list_kernel(int64_t n, const uint32_t* __restrict__ ids,
+                                      const uint8_t* __restrict__ type, Layer4<T> cur,
+                                      Layer4<T> next, Layer4<T> tA, Layer4<T> tB) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = ids[t];
+    const uint8_t ty = type[i];
+    const bool bv = ty == NODE_BOUND || ty == NODE_VALVE;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        T v = cur.f[f][i];
+        tA.f[f][i] = v;
+        tB.f[f][i] = v;
+        if (bv) next.f[f][i] = v;
+    }
+}
+
 // EvalDivError (TimeLayer3D.h:595-641): per NODE_IN node with i < nx-1, j < ny-1,
 // k < nz-1, |flux divergence| of the dual cell; field-type sums, double result.
 // Grid-stride loop, one (sum, count) partial per block, summed in block order by
@@ -187,6 +222,24 @@ void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const
 }
 
 template <class T>
+void launch_update_boundaries_list(int64_t n, const uint32_t* ids, const T* const* gval,
+                                   Layer4<T> cur, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((update_boundaries_list_kernel<T>), dim3(blocks_for(n)), dim3(kBS), 0, st, n,
+                       ids, gval[0], gval[1], gval[2], gval[3], cur);
+    CMCX_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_prep_step_list(int64_t n, const uint32_t* ids, const uint8_t* type, Layer4<T> cur,
+                           Layer4<T> next, Layer4<T> tA, Layer4<T> tB, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((prep_step_list_kernel<T>), dim3(blocks_for(n)), dim3(kBS), 0, st, n, ids,
+                       type, cur, next, tA, tB);
+    CMCX_LAUNCH_CHECK();
+}
+
+template <class T>
 void launch_prep_step(const Geom& g, const uint8_t* type, Layer4<T> cur, Layer4<T> next,
                       Layer4<T> tA, Layer4<T> tB, hipStream_t st) {
     int full = tB.f[0] == nullptr ? 1 : 0;
@@ -250,6 +303,10 @@ void launch_mask_check(const Geom& g, const uint8_t* type, unsigned long long* d
 #define INST(T)                                                                                   \
     template void launch_update_boundaries<T>(const Geom&, const uint8_t*, const T* const*,        \
                                               Layer4<T>, hipStream_t);                             \
+    template void launch_update_boundaries_list<T>(int64_t, const uint32_t*, const T* const*,      \
+                                                   Layer4<T>, hipStream_t);                        \
+    template void launch_prep_step_list<T>(int64_t, const uint32_t*, const uint8_t*, Layer4<T>,    \
+                                           Layer4<T>, Layer4<T>, Layer4<T>, hipStream_t);          \
     template void launch_prep_step<T>(const Geom&, const uint8_t*, Layer4<T>, Layer4<T>,           \
                                       Layer4<T>, Layer4<T>, hipStream_t);                          \
     template void launch_div_error<T>(const Geom&, const uint8_t*, Layer4<T>, T, T, T, double*,    \

@@ -379,14 +379,12 @@ __device__ __forceinline__ void solve_reduced_lines(T* red, RedLayout L, int nl,
 }
 
 // ---------------------------------------------------------------------------
-// Largest work-group of the strided kernel: FP32 1024 threads (128 VGPRs per
-// lane), FP64 512 threads (256 VGPRs; the chunk state takes twice the registers).
-template <class T>
-constexpr int strided_max_threads() { return sizeof(T) == 8 ? 512 : 1024; }
-
-// Strided lines (X or Y sweep).
-template <class T, int AXIS, int M, int LPG>
-__global__ void __launch_bounds__(strided_max_threads<T>()) fused_strided_kernel(
+// Strided lines (X or Y sweep).  A work-group owns LPG consecutive z lines and
+// every row of them (MAXT = 32 * LPG threads for lines of up to 256 rows in
+// chunks of M = 8); MINW is the occupancy the register allocation must allow
+// (waves per SIMD), so that two half-width work-groups can share a CU.
+template <class T, int AXIS, int M, int LPG, int MAXT, int MINW>
+__global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
     const uint8_t* __restrict__ bct, const T* __restrict__ gv0, const T* __restrict__ gv1,
     const T* __restrict__ gv2, const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin,
@@ -497,8 +495,8 @@ __device__ __forceinline__ void store_run(T* __restrict__ p, const T* r, bool ve
 // vector accesses (the wave covers its two lines with back-to-back vectors);
 // the along-line neighbours of a chunk come from the adjacent lanes by shuffle
 // and the cross-line derivatives of w from vector loads of the neighbour lines.
-template <class T, int M>
-__global__ void __launch_bounds__(256) fused_z_kernel(
+template <class T, int M, int MINW>
+__global__ void __launch_bounds__(256, MINW) fused_z_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
     const uint8_t* __restrict__ bct, const T* __restrict__ gv0, const T* __restrict__ gv1,
     const T* __restrict__ gv2, const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin,
@@ -647,13 +645,16 @@ __global__ void __launch_bounds__(256) fused_z_kernel(
 
 constexpr int kChunk = 8;  // rows per chunk of the strided kernel
 
+// Lines per work-group of a strided-sweep variant: 0 = full 128-byte rows
+// (32 * LPG threads, one work-group per CU), 1 = half rows of 64 bytes with
+// registers capped for two work-groups per CU (two work-groups share each
+// cache line).
 template <class T>
-constexpr int lanes_per_group() { return 128 / (int)sizeof(T); }
+constexpr int variant_lpg(int variant) { return (variant == 0 ? 128 : 64) / (int)sizeof(T); }
 
-template <class T>
-int strided_waves(int n) {
-    constexpr int GPW = 64 / lanes_per_group<T>();
-    return (n + kChunk * GPW - 1) / (kChunk * GPW);
+int strided_waves_for(int n, int lpg) {
+    const int gpw = 64 / lpg;
+    return (n + kChunk * gpw - 1) / (kChunk * gpw);
 }
 
 int z_chunk(int n) {
@@ -665,17 +666,27 @@ int z_chunk(int n) {
 }  // namespace
 
 template <class T>
+int strided_lpg(int variant) {
+    return variant_lpg<T>(variant);
+}
+
+template <class T>
 bool fused_supported(const Geom& g) {
-    const int maxw = strided_max_threads<T>() / 64;
     if (z_chunk(g.nz) == 0 || (sizeof(T) == 8 && z_chunk(g.nz) > 8)) return false;
-    return strided_waves<T>(g.nx) <= maxw && strided_waves<T>(g.ny) <= maxw;
+    // both variants: at most 32 * LPG threads
+    for (int v = 0; v < 2; v++) {
+        const int lpg = variant_lpg<T>(v);
+        if (strided_waves_for(g.nx, lpg) * 64 > 32 * lpg) return false;
+        if (strided_waves_for(g.ny, lpg) * 64 > 32 * lpg) return false;
+    }
+    return true;
 }
 
 template <class T>
 void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                         const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
                         Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-                        const int32_t* lines, int64_t nlines, hipStream_t st) {
+                        const int32_t* lines, int64_t nlines, int variant, hipStream_t st) {
     if (nlines == 0) return;
     if (axis == AX_Z) {
         const int waves = 4;
@@ -686,29 +697,46 @@ void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const 
         unsigned blocks = (unsigned)((nlines + 2 * waves - 1) / (2 * waves));
         // 16-byte vectors need every line to start on a 16-byte boundary
         const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
-#define LZ(MM)                                                                                   \
-    hipLaunchKernelGGL((fused_z_kernel<T, MM>), dim3(blocks), dim3(64 * waves), lds, st, g, p, \
-                       type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, tin, out, tout,  \
-                       merges, lines, (int)nlines, vec_ok)
-        if (m == 2) LZ(2);
-        else if (m == 4) LZ(4);
-        else if (m == 8) LZ(8);
-        else LZ(16);
+        // variant 1 caps the registers for 4 (FP32) / 2 (FP64) waves per SIMD
+        constexpr int MW = sizeof(T) == 8 ? 2 : 4;
+#define LZ(MM, W)                                                                                \
+    hipLaunchKernelGGL((fused_z_kernel<T, MM, W>), dim3(blocks), dim3(64 * waves), lds, st, g, \
+                       p, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, tin, out,     \
+                       tout, merges, lines, (int)nlines, vec_ok)
+        if (variant == 0) {
+            if (m == 2) LZ(2, 1);
+            else if (m == 4) LZ(4, 1);
+            else if (m == 8) LZ(8, 1);
+            else LZ(16, 1);
+        } else {
+            if (m == 2) LZ(2, MW);
+            else if (m == 4) LZ(4, MW);
+            else if (m == 8) LZ(8, MW);
+            else LZ(16, 1);
+        }
 #undef LZ
     } else {
-        constexpr int LPG = lanes_per_group<T>();
         const int n = axis == AX_X ? g.nx : g.ny;
-        const int waves = strided_waves<T>(n);
-        const int P = waves * (64 / LPG);
+        constexpr int LW = variant_lpg<T>(0), LN = variant_lpg<T>(1);
+        const int lpg = variant ? LN : LW;
+        const int waves = strided_waves_for(n, lpg);
+        const int P = waves * (64 / lpg);
         const int pitch = kRedFields * 2 * P + 1;
-        size_t lds = (size_t)LPG * pitch * sizeof(T);
-        const int ztiles = (g.nz + LPG - 1) / LPG;
-#define LS(AX)                                                                                   \
-    hipLaunchKernelGGL((fused_strided_kernel<T, AX, kChunk, LPG>), dim3((unsigned)nlines),       \
-                       dim3(64 * waves), lds, st, g, p, type, bcv, bct, gval[0], gval[1],         \
-                       gval[2], gval[3], in, tin, out, tout, merges, lines, ztiles)
-        if (axis == AX_X) LS(AX_X);
-        else LS(AX_Y);
+        size_t lds = (size_t)lpg * pitch * sizeof(T);
+        const int ztiles = (g.nz + lpg - 1) / lpg;
+#define LS(AX, LPGV)                                                                             \
+    hipLaunchKernelGGL((fused_strided_kernel<T, AX, kChunk, LPGV, 32 * LPGV,                      \
+                                             (LPGV == LN ? LPGV / 4 : LPGV / 8)>),                \
+                       dim3((unsigned)nlines), dim3(64 * waves), lds, st, g, p, type, bcv, bct,   \
+                       gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, lines,     \
+                       ztiles)
+        if (axis == AX_X) {
+            if (variant) LS(AX_X, LN);
+            else LS(AX_X, LW);
+        } else {
+            if (variant) LS(AX_Y, LN);
+            else LS(AX_Y, LW);
+        }
 #undef LS
     }
     CMCX_LAUNCH_CHECK();
@@ -779,14 +807,16 @@ int shared_free_axes(const uint8_t* d_type, const uint8_t* d_bcv, const uint8_t*
 
 template bool fused_supported<float>(const Geom&);
 template bool fused_supported<double>(const Geom&);
+template int strided_lpg<float>(int);
+template int strided_lpg<double>(int);
 template void launch_fused_sweep<float>(const Geom&, int, const SweepParams<float>&, const uint8_t*,
                                         const uint8_t*, const uint8_t*, const float* const*,
                                         Layer4<float>, Layer4<float>, Layer4<float>, Layer4<float>,
-                                        int, const int32_t*, int64_t, hipStream_t);
+                                        int, const int32_t*, int64_t, int, hipStream_t);
 template void launch_fused_sweep<double>(const Geom&, int, const SweepParams<double>&,
                                          const uint8_t*, const uint8_t*, const uint8_t*,
                                          const double* const*, Layer4<double>, Layer4<double>,
                                          Layer4<double>, Layer4<double>, int, const int32_t*,
-                                         int64_t, hipStream_t);
+                                         int64_t, int, hipStream_t);
 
 }  // namespace cmcx

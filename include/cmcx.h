@@ -169,6 +169,17 @@ int cmcx_set_boundary(cmcx_solver* s, const uint8_t* bc_vel, const uint8_t* bc_t
 int cmcx_profile(cmcx_solver* s, int enable);
 /* Accumulated time of the sweep kernels since the last reset: total ms and launches. */
 int cmcx_profile_read(cmcx_solver* s, double* sweep_ms, int64_t* sweep_launches, int reset);
+/* Autotuner (no reference counterpart): times the fused path's kernel variants
+   of the X, Y and Z sweeps on this domain (writing scratch layers only; the state is
+   untouched), keeps the fastest per axis and caches the choice process-wide for
+   the grid shape and field type.  choice (may be NULL) receives 3 ints. */
+int cmcx_autotune(cmcx_solver* s, int reps, int* choice);
+/* Forces the fused path's kernel variant of one axis.  X (0) and Y (1) sweeps:
+   0 = 128-byte z rows per work-group (one work-group per CU), 1 = 64-byte rows
+   with registers capped for two work-groups per CU.  Z sweep (2): 0 = register
+   use left to the compiler, 1 = capped for more waves per SIMD.  No reference
+   counterpart (tuning knob). */
+int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant);
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
 int cmcx_synchronize(cmcx_solver* s);

@@ -82,10 +82,10 @@ def test_segments_blocked_order(nb):
     s.close()
 
 
-def _run_pair(d, dtype, path, steps, G=2, L=1, merge_order=0, oracle_dtype=None):
+def _run_pair(d, dtype, path, steps, G=2, L=1, merge_order=0, oracle_dtype=None, variant=None):
     ref = oracle_lib.Oracle3D(d, oracle_dtype or dtype, merge_mode=merge_order)
     ref.create_segments()
-    s = make_solver(d, dtype, path=path, merge_order=merge_order)
+    s = make_solver(d, dtype, path=path, merge_order=merge_order, variant=variant)
     errs = []
     for i in range(steps):
         ref.update_boundaries()
@@ -207,14 +207,16 @@ def test_mask_rejected():
         make_solver(d, "f32")
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
-def test_fused_line_lengths(n):
+def test_fused_line_lengths(n, variant):
     """Fused kernels at every line length of the benchmark workloads (chunk and
-    tile padding), FP64 against the FP64 oracle, FP32 against the FP64 oracle."""
+    tile padding), both tile variants, FP64 against the FP64 oracle, FP32
+    against the FP64 oracle."""
     d = domains.lid_sphere(n, radius=n / 6.0, dt=1e-3)
-    say(f"[fused length] {n}^3")
+    say(f"[fused length] {n}^3 variant {variant}")
     for dtype, tol in (("f64", 1e-11), ("f32", 1e-5)):
-        s, ref, _ = _run_pair(d, dtype, FUSED, steps=2, oracle_dtype="f64")
+        s, ref, _ = _run_pair(d, dtype, FUSED, steps=2, oracle_dtype="f64", variant=variant)
         got = s.fields()
         want = ref.fields()
         vs = vel_norm(*want[:3])
@@ -236,4 +238,43 @@ def test_fused_anisotropic_grid():
     for k in range(3):
         assert rel_l2_scaled(got[k], want[k], vs) < 1e-11
     assert rel_l2(got[4], want[3]) < 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fused_variant_ellipsoids(variant):
+    d = CASES["ellipsoids_40"]
+    say(f"[fused variant] ellipsoids_40 variant {variant}")
+    s, ref, _ = _run_pair(d, "f64", FUSED, steps=3, variant=variant)
+    got = s.fields()
+    want = ref.fields()
+    vs = vel_norm(*want[:3])
+    for k in range(3):
+        assert rel_l2_scaled(got[k], want[k], vs) < 1e-11
+    assert rel_l2(got[4], want[3]) < 1e-11
+    s.close()
+
+
+def test_autotune_keeps_state_and_parity():
+    d = CASES["random_free_17x13x21"]
+    say("[autotune]")
+    ref = oracle_lib.Oracle3D(d, "f64")
+    ref.create_segments()
+    s = make_solver(d, "f64", path=FUSED)
+    before = s.fields()
+    choice = s.autotune(2)
+    assert len(choice) == 3 and all(c in (0, 1) for c in choice)
+    after = s.fields()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    for i in range(2):
+        ref.update_boundaries()
+        ref.step(d.dt, 2, 1, True)
+        s.UpdateBoundaries()
+        s.TimeStep(d.dt, 2, 1, True)
+    got = s.fields()
+    want = ref.fields()
+    vs = vel_norm(*want[:3])
+    for k in range(3):
+        assert rel_l2_scaled(got[k], want[k], vs) < 1e-11
     s.close()
