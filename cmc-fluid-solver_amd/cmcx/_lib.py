@@ -78,6 +78,12 @@ SIGNATURES = [
     ("cmcx_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("cmcx_set_tile_variant", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     ("cmcx_synchronize", ctypes.c_int, [_vp]),
+    ("cmcx_create_batch", ctypes.c_int, [ctypes.POINTER(cmcx_domain), ctypes.POINTER(cmcx_params),
+                                         ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    ("cmcx_batch_size", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    ("cmcx_select", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("cmcx_grid_errors", ctypes.c_int, [_vp, _dp]),
+    ("cmcx_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
     ("cmcx2d_create", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params), ctypes.c_int,
                                      _vp, ctypes.POINTER(_vp)]),
     ("cmcx2d_update_boundaries", ctypes.c_int, [_vp]),

@@ -198,6 +198,10 @@ class AdiSolver3D:
     def set_tile_variant(self, axis: int, variant: int):
         check(lib().cmcx_set_tile_variant(self.h, axis, variant), "cmcx_set_tile_variant")
 
+    def set_graph(self, enable: bool = True):
+        """Replay each step as a HIP graph (captured on first use)."""
+        check(lib().cmcx_set_graph(self.h, 1 if enable else 0), "cmcx_set_graph")
+
     def synchronize(self):
         check(lib().cmcx_synchronize(self.h), "cmcx_synchronize")
 
@@ -211,6 +215,72 @@ class AdiSolver3D:
             self.close()
         except Exception:
             pass
+
+
+def _domain_struct(grid, keep: list):
+    arrs = [_u8(grid.type), _u8(grid.bc_vel), _u8(grid.bc_temp)]
+    vals = [_f64(grid.vx), _f64(grid.vy), _f64(grid.vz), _f64(grid.T)]
+    keep.extend(arrs + vals)
+    dom = _lib.cmcx_domain()
+    dom.dims[:] = [grid.nx, grid.ny, grid.nz]
+    dom.h[:] = [grid.dx, grid.dy, grid.dz]
+    dom.base_t = grid.base_t
+    dom.type, dom.bc_vel, dom.bc_temp = (a.ctypes.data_as(_lib._u8p) for a in arrs)
+    for c in range(3):
+        dom.v[c] = vals[c].ctypes.data_as(_lib._dp)
+    dom.T = vals[3].ctypes.data_as(_lib._dp)
+    return dom
+
+
+class BatchSolver3D(AdiSolver3D):
+    """Several independent grids advanced together (one fused launch per sweep
+    covers all of them).  Per-grid calls (fields, segments, GetLayer, ...)
+    address the grid chosen with :meth:`select`."""
+
+    def InitBatch(self, grids, params_list, stream=None):
+        if len(grids) != len(params_list) or not grids:
+            raise ValueError("one FluidParams per grid")
+        keep = []
+        doms = (_lib.cmcx_domain * len(grids))(*[_domain_struct(g, keep) for g in grids])
+        prms = (_lib.cmcx_params * len(grids))(
+            *[_lib.cmcx_params(p.v_T, p.v_vis, p.t_vis, p.t_phi) for p in params_list])
+        h = ctypes.c_void_p(None)
+        check(lib().cmcx_create_batch(doms, prms, len(grids), self.dtype, ctypes.c_void_p(stream),
+                                      ctypes.byref(h)), "cmcx_create_batch")
+        self.h = h
+        self._keep = keep
+        self.grids = list(grids)
+        self.grid = self.grids[0]
+        self.params = params_list[0]
+        return self
+
+    def __len__(self):
+        n = ctypes.c_int(0)
+        check(lib().cmcx_batch_size(self.h, ctypes.byref(n)), "cmcx_batch_size")
+        return n.value
+
+    def select(self, i: int):
+        check(lib().cmcx_select(self.h, int(i)), "cmcx_select")
+        self.grid = self.grids[i]
+        return self
+
+    def grid_errors(self) -> np.ndarray:
+        out = np.zeros(len(self.grids), np.float64)
+        check(lib().cmcx_grid_errors(self.h, out.ctypes.data_as(_lib._dp)), "cmcx_grid_errors")
+        return out
+
+    def fields_of(self, i: int):
+        return self.select(i).fields()
+
+
+def make_batch(doms, dtype: str = "f32", path: int = _lib.CMCX_PATH_FUSED,
+               params_list=None) -> BatchSolver3D:
+    """InitBatch + SetOptions + CreateSegments for a list of domains."""
+    s = BatchSolver3D(dtype)
+    s.InitBatch(doms, params_list or [d.params() for d in doms])
+    s.set_options(path, _lib.CMCX_MERGE_CPU)
+    s.CreateSegments()
+    return s
 
 
 def make_solver(dom, dtype: str = "f32", path: int = _lib.CMCX_PATH_PLAIN,

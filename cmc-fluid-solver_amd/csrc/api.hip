@@ -43,6 +43,7 @@ struct ISolver {
     virtual int get_segment_nodes(int axis, uint8_t* flags, double* vals, int64_t cap) = 0;
     virtual int update_boundaries() = 0;
     virtual int step(double dt, int G, int L, int compute_err, double* err) = 0;
+    virtual int run(double dt, int steps, int G, int L, double* err) = 0;
     virtual int div_error(double* err) = 0;
     virtual int get_fields(void* u, void* v, void* w, void* p, void* T, bool device) = 0;
     virtual int set_fields(const void* u, const void* v, const void* w, const void* T) = 0;
@@ -55,15 +56,40 @@ struct ISolver {
     virtual int step_bytes(int G, int L, double* bytes) = 0;
     virtual int autotune(int reps, int* choice) = 0;
     virtual int set_tile_variant(int axis, int variant) = 0;
+    virtual int set_graph(int enable) = 0;
+    virtual int batch_size() const = 0;
+    virtual int select(int index) = 0;
+    virtual int grid_errors(double* errs) = 0;
     virtual int synchronize() = 0;
 };
+
+// One grid of a solver: its geometry, parameters, place in the flat arrays and
+// segment lists.  A solver advances all its grids together (a batch); the
+// single-domain interface is a batch of one.
+struct Grid {
+    Geom g{};
+    double h[3] = {0, 0, 0};
+    double base_t = 0;
+    cmcx_params prm{};
+    int64_t off = 0;          // flat index of node (0, 0, 0)
+    SegList segs[3];
+    int free_shared = 0;      // axes with a BC_FREE node shared by two segments
+    double err = 0.0;         // divergence error of the last evaluation
+};
+
+template <class T>
+Layer4<T> shifted(const Layer4<T>& L, int64_t off) {
+    Layer4<T> r;
+    for (int f = 0; f < 4; f++) r.f[f] = L.f[f] ? L.f[f] + off : nullptr;
+    return r;
+}
 
 template <class T>
 class Solver final : public ISolver {
    public:
-    Geom g{};
-    double hx_d, hy_d, hz_d, base_t;
-    cmcx_params prm{};
+    std::vector<Grid> grids;
+    int sel = 0;              // grid addressed by the per-grid calls
+    int64_t NT = 0;           // length of the flat arrays (grids + alignment padding)
     hipStream_t st = nullptr;
     int path = CMCX_PATH_PLAIN;
     int merge_order = CMCX_MERGE_CPU;
@@ -71,23 +97,26 @@ class Solver final : public ISolver {
     bool built = false;
     double diff_error = 0.0;
 
+    std::vector<uint8_t> h_type, h_bcv, h_bct;  // host copies of the flat mask / BC kinds
     uint8_t *d_type = nullptr, *d_bcv = nullptr, *d_bct = nullptr;
     T* d_g[4] = {nullptr, nullptr, nullptr, nullptr};
-    T* buf = nullptr;  // 5 layers x 4 fields x N
+    T* buf = nullptr;  // 5 layers x 4 fields x NT
     Layer4<T> cur{}, next{}, half{}, tA{}, tB{};
-    SegList segs[3];
     T* scratch = nullptr;
     size_t scratch_elems = 0;
-    int32_t* lines[3][kFusedVariants] = {};
-    int64_t nlines[3][kFusedVariants] = {};
-    int variant[3] = {0, 0, 0};  // strided tile width per axis (autotuned)
-    int free_shared = 0;  // axes that need the per-segment kernel (shared free nodes)
-    double* d_acc = nullptr;
+    FusedWork work[3][kFusedVariants];
+    InstDesc<T>* d_desc = nullptr;
+    int variant[3] = {0, 0, 0};  // fused kernel variant per axis (autotuned)
+    double* d_acc = nullptr;     // 2 doubles per grid
     double* d_part = nullptr;
-    // node lists (fixed mask): non-IN nodes, BOUND/VALVE nodes
+    // node lists (fixed mask): non-IN nodes, BOUND/VALVE nodes (flat indices)
     uint32_t* d_nonin = nullptr;
     uint32_t* d_bvlist = nullptr;
     int64_t n_nonin = 0, n_bv = 0;
+    // HIP graphs of whole steps, per (boundary fill, dt, G, L, state buffer)
+    bool graph_mode = false;
+    std::map<std::tuple<int, double, int, int, const void*>, hipGraphExec_t> graphs;
+    hipStream_t cap = nullptr;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -95,81 +124,113 @@ class Solver final : public ISolver {
     double prof_ms = 0.0;
     int64_t prof_launches = 0;
 
-    Solver(const cmcx_domain* d, const cmcx_params* p, hipStream_t stream) {
+    Solver(const cmcx_domain* doms, const cmcx_params* ps, int count, hipStream_t stream) {
         st = stream;
-        g.nx = d->dims[0];
-        g.ny = d->dims[1];
-        g.nz = d->dims[2];
-        g.sy = g.nz;
-        g.sx = (int64_t)g.ny * g.nz;
-        g.N = g.sx * g.nx;
-        hx_d = d->h[0];
-        hy_d = d->h[1];
-        hz_d = d->h[2];
-        base_t = d->base_t;
-        prm = *p;
-        const int64_t N = g.N;
-        CMCX_HIP(hipMalloc((void**)&d_type, N));
-        CMCX_HIP(hipMalloc((void**)&d_bcv, N));
-        CMCX_HIP(hipMalloc((void**)&d_bct, N));
-        for (int f = 0; f < 4; f++) CMCX_HIP(hipMalloc((void**)&d_g[f], sizeof(T) * N));
-        CMCX_HIP(hipMalloc((void**)&buf, sizeof(T) * N * 20));
-        CMCX_HIP(hipMemsetAsync(buf, 0, sizeof(T) * N * 20, st));
+        grids.resize(count);
+        int64_t off = 0;
+        for (int i = 0; i < count; i++) {
+            const cmcx_domain* d = &doms[i];
+            Grid& G = grids[i];
+            G.g.nx = d->dims[0];
+            G.g.ny = d->dims[1];
+            G.g.nz = d->dims[2];
+            G.g.sy = G.g.nz;
+            G.g.sx = (int64_t)G.g.ny * G.g.nz;
+            G.g.N = G.g.sx * G.g.nx;
+            for (int a = 0; a < 3; a++) G.h[a] = d->h[a];
+            G.base_t = d->base_t;
+            G.prm = ps[i];
+            G.off = off;
+            off += (G.g.N + 63) / 64 * 64;  // 256-byte aligned grids
+        }
+        NT = off;
+        if (NT > ((int64_t)1 << 31) - 1) throw std::string("grids too large (total node count must fit in int32)");
+        h_type.assign(NT, (uint8_t)NODE_OUT);  // padding between grids: never read
+        h_bcv.assign(NT, 0);
+        h_bct.assign(NT, 0);
+        std::vector<T> hg[4];
+        for (int f = 0; f < 4; f++) hg[f].assign(NT, (T)0);
+        for (int i = 0; i < count; i++) {
+            const cmcx_domain* d = &doms[i];
+            const int64_t o = grids[i].off, N = grids[i].g.N;
+            std::memcpy(&h_type[o], d->type, N);
+            std::memcpy(&h_bcv[o], d->bc_vel, N);
+            std::memcpy(&h_bct[o], d->bc_temp, N);
+            const double* src[4] = {d->v[0], d->v[1], d->v[2], d->T};
+            for (int f = 0; f < 4; f++)
+                for (int64_t k = 0; k < N; k++) hg[f][o + k] = (T)src[f][k];
+        }
+        CMCX_HIP(hipMalloc((void**)&d_type, NT));
+        CMCX_HIP(hipMalloc((void**)&d_bcv, NT));
+        CMCX_HIP(hipMalloc((void**)&d_bct, NT));
+        for (int f = 0; f < 4; f++) CMCX_HIP(hipMalloc((void**)&d_g[f], sizeof(T) * NT));
+        CMCX_HIP(hipMalloc((void**)&buf, sizeof(T) * NT * 20));
+        CMCX_HIP(hipMemsetAsync(buf, 0, sizeof(T) * NT * 20, st));
         Layer4<T>* Ls[5] = {&cur, &next, &half, &tA, &tB};
         for (int l = 0; l < 5; l++)
-            for (int f = 0; f < 4; f++) Ls[l]->f[f] = buf + (int64_t)(l * 4 + f) * N;
-        CMCX_HIP(hipMalloc((void**)&d_acc, sizeof(double) * 2));
+            for (int f = 0; f < 4; f++) Ls[l]->f[f] = buf + (int64_t)(l * 4 + f) * NT;
+        CMCX_HIP(hipMalloc((void**)&d_acc, sizeof(double) * 2 * count));
         CMCX_HIP(hipMalloc((void**)&d_part, sizeof(double) * div_partials()));
-        CMCX_HIP(hipMemcpyAsync(d_type, d->type, N, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMalloc((void**)&d_desc, sizeof(InstDesc<T>) * count));
+        CMCX_HIP(hipMemcpyAsync(d_type, h_type.data(), NT, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMemcpyAsync(d_bcv, h_bcv.data(), NT, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMemcpyAsync(d_bct, h_bct.data(), NT, hipMemcpyHostToDevice, st));
+        for (int f = 0; f < 4; f++)
+            CMCX_HIP(hipMemcpyAsync(d_g[f], hg[f].data(), sizeof(T) * NT, hipMemcpyHostToDevice, st));
         {
             std::vector<uint32_t> nonin, bvl;
-            for (int64_t i = 0; i < N; i++) {
-                const uint8_t t = d->type[i];
-                if (t != NODE_IN) nonin.push_back((uint32_t)i);
-                if (t == NODE_BOUND || t == NODE_VALVE) bvl.push_back((uint32_t)i);
-            }
+            for (const Grid& G : grids)
+                for (int64_t k = 0; k < G.g.N; k++) {
+                    const int64_t i = G.off + k;
+                    const uint8_t t = h_type[i];
+                    if (t != NODE_IN) nonin.push_back((uint32_t)i);
+                    if (t == NODE_BOUND || t == NODE_VALVE) bvl.push_back((uint32_t)i);
+                }
             n_nonin = (int64_t)nonin.size();
             n_bv = (int64_t)bvl.size();
             CMCX_HIP(hipMalloc((void**)&d_nonin, sizeof(uint32_t) * std::max<int64_t>(n_nonin, 1)));
             CMCX_HIP(hipMalloc((void**)&d_bvlist, sizeof(uint32_t) * std::max<int64_t>(n_bv, 1)));
             if (n_nonin)
-                CMCX_HIP(hipMemcpy(d_nonin, nonin.data(), sizeof(uint32_t) * n_nonin,
-                                   hipMemcpyHostToDevice));
+                CMCX_HIP(hipMemcpyAsync(d_nonin, nonin.data(), sizeof(uint32_t) * n_nonin,
+                                        hipMemcpyHostToDevice, st));
             if (n_bv)
-                CMCX_HIP(hipMemcpy(d_bvlist, bvl.data(), sizeof(uint32_t) * n_bv, hipMemcpyHostToDevice));
+                CMCX_HIP(hipMemcpyAsync(d_bvlist, bvl.data(), sizeof(uint32_t) * n_bv,
+                                        hipMemcpyHostToDevice, st));
+            // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
+            for (int f = 0; f < 4; f++)
+                CMCX_HIP(hipMemcpyAsync(cur.f[f], d_g[f], sizeof(T) * NT, hipMemcpyDeviceToDevice, st));
+            CMCX_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
         }
-        upload_boundary(d->bc_vel, d->bc_temp, d->v[0], d->v[1], d->v[2], d->T);
-        // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
-        for (int f = 0; f < 4; f++)
-            CMCX_HIP(hipMemcpyAsync(cur.f[f], d_g[f], sizeof(T) * N, hipMemcpyDeviceToDevice, st));
-        CMCX_HIP(hipStreamSynchronize(st));
+        upload_desc();
     }
 
-    void upload_boundary(const uint8_t* bv, const uint8_t* bt, const double* vx, const double* vy,
-                         const double* vz, const double* Tt) {
-        const int64_t N = g.N;
-        CMCX_HIP(hipMemcpyAsync(d_bcv, bv, N, hipMemcpyHostToDevice, st));
-        CMCX_HIP(hipMemcpyAsync(d_bct, bt, N, hipMemcpyHostToDevice, st));
-        std::vector<T> tmp(N);
-        const double* src[4] = {vx, vy, vz, Tt};
-        for (int f = 0; f < 4; f++) {
-            for (int64_t i = 0; i < N; i++) tmp[i] = (T)src[f][i];
-            CMCX_HIP(hipMemcpyAsync(d_g[f], tmp.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
-            CMCX_HIP(hipStreamSynchronize(st));  // tmp is reused
+    // Descriptor table of the fused kernels (dt-independent part of SweepParams).
+    void upload_desc() {
+        std::vector<InstDesc<T>> h(grids.size());
+        for (size_t i = 0; i < grids.size(); i++) {
+            h[i].g = grids[i].g;
+            h[i].off = grids[i].off;
+            for (int a = 0; a < 3; a++) h[i].p[a] = sweep_params((int)i, a, (T)1);
         }
+        CMCX_HIP(hipMemcpyAsync(d_desc, h.data(), sizeof(InstDesc<T>) * h.size(), hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));
     }
 
     ~Solver() override {
         hipStreamSynchronize(st);
+        clear_graphs();
+        if (cap) hipStreamDestroy(cap);
         hipFree(d_type);
         hipFree(d_bcv);
         hipFree(d_bct);
         for (auto* p : d_g) hipFree(p);
         hipFree(buf);
-        for (auto& s : segs) hipFree(s.d);
+        for (auto& G : grids)
+            for (auto& s : G.segs) hipFree(s.d);
         hipFree(scratch);
-        for (auto& la : lines)
-            for (auto* l : la) hipFree(l);
+        for (auto& wa : work)
+            for (auto& w : wa) w.release();
+        hipFree(d_desc);
         hipFree(d_acc);
         hipFree(d_part);
         hipFree(d_nonin);
@@ -180,12 +241,14 @@ class Solver final : public ISolver {
         }
     }
 
+    Grid& G() { return grids[sel]; }
+
     int check_mask() {
         unsigned long long* d_out = nullptr;
         unsigned long long h[2] = {0, 0};
         CMCX_HIP(hipMalloc((void**)&d_out, sizeof(h)));
         CMCX_HIP(hipMemsetAsync(d_out, 0, sizeof(h), st));
-        launch_mask_check(g, d_type, d_out, st);
+        for (const Grid& Gr : grids) launch_mask_check(Gr.g, d_type + Gr.off, d_out, st);
         CMCX_HIP(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, st));
         CMCX_HIP(hipStreamSynchronize(st));
         CMCX_HIP(hipFree(d_out));
@@ -198,6 +261,11 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    void clear_graphs() {
+        for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+
     int set_options(int p, int m) override {
         if (p != CMCX_PATH_PLAIN && p != CMCX_PATH_FUSED) return CMCX_ERR_ARG;
         if (m != CMCX_MERGE_CPU && m != CMCX_MERGE_GPU_ORDER) return CMCX_ERR_ARG;
@@ -205,30 +273,43 @@ class Solver final : public ISolver {
             set_error("the fused path implements the CPU merge order only");
             return CMCX_ERR_UNSUPPORTED;
         }
-        if (p == CMCX_PATH_FUSED && !fused_supported<T>(g)) {
-            set_error("fused path: grid lines longer than supported");
-            return CMCX_ERR_UNSUPPORTED;
-        }
+        if (p == CMCX_PATH_FUSED)
+            for (const Grid& Gr : grids)
+                if (!fused_supported<T>(Gr.g)) {
+                    set_error("fused path: grid lines longer than supported");
+                    return CMCX_ERR_UNSUPPORTED;
+                }
         path = p;
         merge_order = m;
+        clear_graphs();
         if (built) prepare_path_buffers();
         return CMCX_OK;
     }
 
     int set_nblockz(int nb) override {
-        if (nb < 1 || nb > g.nz || nb > 64) return CMCX_ERR_ARG;
+        for (const Grid& Gr : grids)
+            if (nb < 1 || nb > Gr.g.nz || nb > 64) return CMCX_ERR_ARG;
         nblockz = nb;
         built = false;
+        clear_graphs();
         return CMCX_OK;
     }
 
-    bool plain_axis(int a) const { return path == CMCX_PATH_PLAIN || ((free_shared >> a) & 1); }
+    // The per-segment kernel runs an axis on the plain path, and for every axis
+    // on which some grid has a BC_FREE node shared by two segments.
+    bool plain_axis(int a) const {
+        if (path == CMCX_PATH_PLAIN) return true;
+        for (const Grid& Gr : grids)
+            if ((Gr.free_shared >> a) & 1) return true;
+        return false;
+    }
 
     void prepare_path_buffers() {
         size_t need = 0;
         for (int a = 0; a < 3; a++)
             if (plain_axis(a))
-                need = std::max(need, (size_t)segs[a].max_size * 4 * (size_t)segs[a].n * 2);
+                for (const Grid& Gr : grids)
+                    need = std::max(need, (size_t)Gr.segs[a].max_size * 4 * (size_t)Gr.segs[a].n * 2);
         if (need > scratch_elems) {
             hipFree(scratch);
             scratch = nullptr;
@@ -237,18 +318,26 @@ class Solver final : public ISolver {
         }
         if (path == CMCX_PATH_FUSED)
             for (int a = 0; a < 3; a++)
-                if (!plain_axis(a)) ensure_lines(a, a == AX_Z ? 0 : variant[a]);
+                if (!plain_axis(a)) ensure_work(a, a == AX_Z ? 0 : variant[a]);
+        clear_graphs();
     }
 
-    void ensure_lines(int a, int v) {
-        if (lines[a][v] || nlines[a][v]) return;
-        build_active_lines(d_type, g, a, strided_lpg<T>(v), &lines[a][v], &nlines[a][v], st);
-        if (nlines[a][v] == 0) nlines[a][v] = -1;  // built, empty
+    void ensure_work(int a, int v) {
+        if (work[a][v].maxn) return;  // built
+        std::vector<Geom> gs;
+        std::vector<int64_t> offs;
+        for (const Grid& Gr : grids) {
+            gs.push_back(Gr.g);
+            offs.push_back(Gr.off);
+        }
+        build_fused_work(h_type.data(), gs, offs, a, strided_lpg<T>(v), sizeof(T), work[a][v]);
     }
 
     int build_segments() override {
-        for (int a = 0; a < 3; a++) build_segment_list(d_type, g, a, nblockz, segs[a], st);
-        free_shared = shared_free_axes(d_type, d_bcv, d_bct, g, st);
+        for (Grid& Gr : grids) {
+            for (int a = 0; a < 3; a++) build_segment_list(d_type + Gr.off, Gr.g, a, nblockz, Gr.segs[a], st);
+            Gr.free_shared = shared_free_axes(&h_type[Gr.off], &h_bcv[Gr.off], &h_bct[Gr.off], Gr.g);
+        }
         auto it = tune_cache().find(tune_key());
         if (it != tune_cache().end())
             for (int a = 0; a < 3; a++) variant[a] = it->second[a];
@@ -260,14 +349,14 @@ class Solver final : public ISolver {
     int num_segments(int axis, int64_t* n) override {
         if (axis < 0 || axis > 2 || !n) return CMCX_ERR_ARG;
         if (!built) return CMCX_ERR_STATE;
-        *n = segs[axis].n;
+        *n = G().segs[axis].n;
         return CMCX_OK;
     }
 
     int get_segments(int axis, int32_t* out, int64_t cap) override {
         if (axis < 0 || axis > 2 || !out) return CMCX_ERR_ARG;
         if (!built) return CMCX_ERR_STATE;
-        const SegList& s = segs[axis];
+        const SegList& s = G().segs[axis];
         if (cap < s.n) return CMCX_ERR_ARG;
         if (s.n == 0) return CMCX_OK;
         std::vector<int32_t> h(7 * s.n);
@@ -281,12 +370,14 @@ class Solver final : public ISolver {
     int get_segment_nodes(int axis, uint8_t* flags, double* vals, int64_t cap) override {
         if (axis < 0 || axis > 2 || !flags || !vals) return CMCX_ERR_ARG;
         if (!built) return CMCX_ERR_STATE;
-        int64_t n = segs[axis].n;
+        const Grid& Gr = G();
+        const Geom& g = Gr.g;
+        int64_t n = Gr.segs[axis].n;
         if (cap < n) return CMCX_ERR_ARG;
         std::vector<int32_t> sl(7 * std::max<int64_t>(n, 1));
         if (n) {
             std::vector<int32_t> h(7 * n);
-            CMCX_HIP(hipMemcpyAsync(h.data(), segs[axis].d, sizeof(int32_t) * 7 * n,
+            CMCX_HIP(hipMemcpyAsync(h.data(), Gr.segs[axis].d, sizeof(int32_t) * 7 * n,
                                     hipMemcpyDeviceToHost, st));
             CMCX_HIP(hipStreamSynchronize(st));
             for (int64_t i = 0; i < n; i++)
@@ -294,14 +385,11 @@ class Solver final : public ISolver {
         }
         // node records from the device copy of the grid
         const int64_t N = g.N;
-        std::vector<uint8_t> t(N), bv(N), bt(N);
         std::vector<T> gv[4];
-        CMCX_HIP(hipMemcpyAsync(t.data(), d_type, N, hipMemcpyDeviceToHost, st));
-        CMCX_HIP(hipMemcpyAsync(bv.data(), d_bcv, N, hipMemcpyDeviceToHost, st));
-        CMCX_HIP(hipMemcpyAsync(bt.data(), d_bct, N, hipMemcpyDeviceToHost, st));
         for (int f = 0; f < 4; f++) {
             gv[f].resize(N);
-            CMCX_HIP(hipMemcpyAsync(gv[f].data(), d_g[f], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+            CMCX_HIP(hipMemcpyAsync(gv[f].data(), d_g[f] + Gr.off, sizeof(T) * N,
+                                    hipMemcpyDeviceToHost, st));
         }
         CMCX_HIP(hipStreamSynchronize(st));
         for (int64_t i = 0; i < n; i++) {
@@ -309,9 +397,9 @@ class Solver final : public ISolver {
             int64_t ids[2] = {q[0] * g.sx + q[1] * g.sy + q[2], q[3] * g.sx + q[4] * g.sy + q[5]};
             for (int e = 0; e < 2; e++) {
                 int64_t id = ids[e];
-                flags[6 * i + 3 * e + 0] = t[id];
-                flags[6 * i + 3 * e + 1] = bv[id];
-                flags[6 * i + 3 * e + 2] = bt[id];
+                flags[6 * i + 3 * e + 0] = h_type[Gr.off + id];
+                flags[6 * i + 3 * e + 1] = h_bcv[Gr.off + id];
+                flags[6 * i + 3 * e + 2] = h_bct[Gr.off + id];
                 for (int f = 0; f < 4; f++) vals[8 * i + 4 * e + f] = (double)gv[f][id];
             }
         }
@@ -323,18 +411,19 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
-    SweepParams<T> sweep_params(int axis, T dt) const {
+    SweepParams<T> sweep_params(int gi, int axis, T dt) const {
+        const Grid& Gr = grids[gi];
         SweepParams<T> p;
         p.dt = dt;
-        p.hx = (T)hx_d;
-        p.hy = (T)hy_d;
-        p.hz = (T)hz_d;
+        p.hx = (T)Gr.h[0];
+        p.hy = (T)Gr.h[1];
+        p.hz = (T)Gr.h[2];
         p.h = axis == AX_X ? p.hx : axis == AX_Y ? p.hy : p.hz;
-        T v_vis = (T)prm.v_vis, t_vis = (T)prm.t_vis;
+        T v_vis = (T)Gr.prm.v_vis, t_vis = (T)Gr.prm.t_vis;
         p.vis_v = v_vis / (p.h * p.h);
         p.vis_t = t_vis / (p.h * p.h);
-        p.v_T = (T)prm.v_T;
-        p.t_phi = (T)prm.t_phi;
+        p.v_T = (T)Gr.prm.v_T;
+        p.t_phi = (T)Gr.prm.t_phi;
         p.three_dt = (T)3 / dt;
         p.inv_2h = (T)1 / (2 * p.h);
         p.inv_2hx = (T)1 / (2 * p.hx);
@@ -366,34 +455,33 @@ class Solver final : public ISolver {
     }
 
     void sweep(int axis, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-               int var0, int var1, const SweepParams<T>& p) {
+               int var0, int var1, T dt) {
         hipEvent_t* ev = nullptr;
         if (prof) {
             if (ev_used >= 4096) harvest_events();
             ev = next_events();
             CMCX_HIP(hipEventRecord(ev[0], st));
         }
-        if (plain_axis(axis))
-            launch_plain_sweep<T>(g, axis, segs[axis], p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout,
-                                  merges, var0, var1, scratch, st);
-        else
-        {
+        if (plain_axis(axis)) {
+            for (int i = 0; i < (int)grids.size(); i++) {
+                const Grid& Gr = grids[i];
+                const int64_t o = Gr.off;
+                const T* gs[4] = {d_g[0] + o, d_g[1] + o, d_g[2] + o, d_g[3] + o};
+                launch_plain_sweep<T>(Gr.g, axis, Gr.segs[axis], sweep_params(i, axis, dt), d_type + o,
+                                      d_bcv + o, d_bct + o, gs, shifted(in, o), shifted(tin, o),
+                                      shifted(out, o), shifted(tout, o), merges, var0, var1, scratch, st);
+            }
+        } else {
             const int v = variant[axis];
             const int lv = axis == AX_Z ? 0 : v;
-            launch_fused_sweep<T>(g, axis, p, d_type, d_bcv, d_bct, d_g, in, tin, out, tout, merges,
-                                  lines[axis][lv], std::max<int64_t>(nlines[axis][lv], 0), v, st);
+            launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_type, d_bcv, d_bct, d_g,
+                                  in, tin, out, tout, merges, v, st);
         }
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
 
-    int step(double dt_d, int G, int L, int compute_err, double* err) override {
-        if (!built) {
-            set_error("step before build_segments");
-            return CMCX_ERR_STATE;
-        }
-        if (G < 0 || L < 1) return CMCX_ERR_ARG;
-        const T dt = (T)dt_d;
-        SweepParams<T> P[3] = {sweep_params(AX_X, dt), sweep_params(AX_Y, dt), sweep_params(AX_Z, dt)};
+    // Enqueues the sweeps of one time step (no synchronisation, no host reads).
+    void enqueue_step(T dt, int G, int L) {
         if (merge_order == CMCX_MERGE_CPU) {
             launch_prep_step_list<T>(n_nonin, d_nonin, d_type, cur, next, tA, tB, st);
             Layer4<T> tin = cur;  // temp == cur at the first sweep
@@ -407,7 +495,7 @@ class Solver final : public ISolver {
                     for (int l = 0; l < L; l++) {
                         Layer4<T> tout = other(tin);
                         int merges = (a == AX_X && l == L - 1) ? 2 : 1;
-                        sweep(a, ins[q], tin, outs[q], tout, merges, 0, 4, P[a]);
+                        sweep(a, ins[q], tin, outs[q], tout, merges, 0, 4, dt);
                         tin = tout;
                     }
                 }
@@ -415,7 +503,9 @@ class Solver final : public ISolver {
         } else {
             // the reference's CUDA order: temp <- cur, then per field in-place merges
             Layer4<T> none{};
-            launch_prep_step<T>(g, d_type, cur, next, tA, none, st);
+            Geom flat{};
+            flat.N = NT;
+            launch_prep_step<T>(flat, d_type, cur, next, tA, none, st);
             for (int it = 0; it < G; it++) {
                 const int axes[3] = {AX_Z, AX_Y, AX_X};
                 Layer4<T> ins[3] = {cur, next, half};
@@ -423,8 +513,54 @@ class Solver final : public ISolver {
                 for (int q = 0; q < 3; q++)
                     for (int l = 0; l < L; l++)
                         for (int v = 0; v < 4; v++)
-                            sweep(axes[q], ins[q], tA, outs[q], tA, 1, v, v + 1, P[axes[q]]);
+                            sweep(axes[q], ins[q], tA, outs[q], tA, 1, v, v + 1, dt);
             }
+        }
+    }
+
+    // The graph of (boundary fill +) one step for the current state buffer;
+    // captured on a private stream the first time, replayed on the solver's stream.
+    hipGraphExec_t step_graph(bool ub, T dt, int G, int L) {
+        auto key = std::make_tuple(ub ? 1 : 0, (double)dt, G, L, (const void*)cur.f[0]);
+        auto it = graphs.find(key);
+        if (it != graphs.end()) return it->second;
+        if (!cap) CMCX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipStream_t user = st;
+        hipGraph_t gr = nullptr;
+        st = cap;
+        CMCX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            if (ub) update_boundaries();
+            enqueue_step(dt, G, L);
+        } catch (...) {
+            hipStreamEndCapture(cap, &gr);
+            if (gr) hipGraphDestroy(gr);
+            st = user;
+            throw;
+        }
+        st = user;
+        CMCX_HIP(hipStreamEndCapture(cap, &gr));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        CMCX_HIP(e);
+        graphs[key] = ex;
+        return ex;
+    }
+
+    // Launches the step (graph or direct), then the divergence guard and swap.
+    int advance(bool ub, double dt_d, int G, int L, int compute_err, double* err) {
+        if (!built) {
+            set_error("step before build_segments");
+            return CMCX_ERR_STATE;
+        }
+        if (G < 0 || L < 1) return CMCX_ERR_ARG;
+        const T dt = (T)dt_d;
+        if (graph_mode && !prof) {
+            CMCX_HIP(hipGraphLaunch(step_graph(ub, dt, G, L), st));
+        } else {
+            if (ub) update_boundaries();
+            enqueue_step(dt, G, L);
         }
         if (compute_err) diff_error = eval_div(next);
         if (err) *err = diff_error;
@@ -437,35 +573,68 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    int step(double dt, int G, int L, int compute_err, double* err) override {
+        return advance(false, dt, G, L, compute_err, err);
+    }
+
+    // The reference's time loop (FluidSolver3D.cpp:226-242): boundary fill and
+    // step, error every 10th step and on the last.
+    int run(double dt, int steps, int G, int L, double* err) override {
+        for (int i = 0; i < steps; i++) {
+            int rc = advance(true, dt, G, L, (i % 10 == 0) || (i == steps - 1), err);
+            if (rc != CMCX_OK) return rc;
+        }
+        return CMCX_OK;
+    }
+
+    // Divergence error of every grid of layer Lr (stored per grid); returns the largest.
     double eval_div(Layer4<T> Lr) {
-        launch_div_error<T>(g, d_type, Lr, (T)hx_d, (T)hy_d, (T)hz_d, d_acc, d_part, st);
-        double h[2];
-        CMCX_HIP(hipMemcpyAsync(h, d_acc, sizeof(h), hipMemcpyDeviceToHost, st));
+        const int n = (int)grids.size();
+        for (int i = 0; i < n; i++) {
+            const Grid& Gr = grids[i];
+            launch_div_error<T>(Gr.g, d_type + Gr.off, shifted(Lr, Gr.off), (T)Gr.h[0], (T)Gr.h[1],
+                                (T)Gr.h[2], d_acc + 2 * i, d_part, st);
+        }
+        std::vector<double> h(2 * n);
+        CMCX_HIP(hipMemcpyAsync(h.data(), d_acc, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
         CMCX_HIP(hipStreamSynchronize(st));
-        return h[0] / h[1];
+        double worst = 0.0;
+        for (int i = 0; i < n; i++) {
+            grids[i].err = h[2 * i] / h[2 * i + 1];
+            worst = i == 0 ? grids[i].err : std::max(worst, grids[i].err);
+        }
+        return worst;
     }
 
     int div_error(double* err) override {
         if (!err) return CMCX_ERR_ARG;
-        *err = eval_div(cur);
+        eval_div(cur);
+        *err = G().err;
+        return CMCX_OK;
+    }
+
+    int grid_errors(double* errs) override {
+        if (!errs) return CMCX_ERR_ARG;
+        for (size_t i = 0; i < grids.size(); i++) errs[i] = grids[i].err;
         return CMCX_OK;
     }
 
     int get_fields(void* u, void* v, void* w, void* p, void* Tt, bool device) override {
-        const int64_t N = g.N;
+        const Grid& Gr = G();
+        const int64_t N = Gr.g.N, o = Gr.off;
         hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
         void* dst[4] = {u, v, w, Tt};
         for (int f = 0; f < 4; f++)
-            if (dst[f]) CMCX_HIP(hipMemcpyAsync(dst[f], cur.f[f], sizeof(T) * N, k, st));
+            if (dst[f]) CMCX_HIP(hipMemcpyAsync(dst[f], cur.f[f] + o, sizeof(T) * N, k, st));
         if (p) {
             if (device) {
-                launch_scale<T>(cur.f[3], (T*)p, (T)prm.v_T, N, st);
+                launch_scale<T>(cur.f[3] + o, (T*)p, (T)Gr.prm.v_T, N, st);
             } else {
                 // p = v_T * T, the ideal-gas closure the reference's momentum rows use
                 std::vector<T> tmp(N);
-                CMCX_HIP(hipMemcpyAsync(tmp.data(), cur.f[3], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+                CMCX_HIP(hipMemcpyAsync(tmp.data(), cur.f[3] + o, sizeof(T) * N, hipMemcpyDeviceToHost, st));
                 CMCX_HIP(hipStreamSynchronize(st));
-                T vT = (T)prm.v_T;
+                T vT = (T)Gr.prm.v_T;
                 T* pp = (T*)p;
                 for (int64_t i = 0; i < N; i++) pp[i] = vT * tmp[i];
             }
@@ -475,33 +644,38 @@ class Solver final : public ISolver {
     }
 
     int set_fields(const void* u, const void* v, const void* w, const void* Tt) override {
+        const Grid& Gr = G();
         const void* src[4] = {u, v, w, Tt};
         for (int f = 0; f < 4; f++)
             if (src[f])
-                CMCX_HIP(hipMemcpyAsync(cur.f[f], src[f], sizeof(T) * g.N, hipMemcpyHostToDevice, st));
+                CMCX_HIP(hipMemcpyAsync(cur.f[f] + Gr.off, src[f], sizeof(T) * Gr.g.N,
+                                        hipMemcpyHostToDevice, st));
         CMCX_HIP(hipStreamSynchronize(st));
         return CMCX_OK;
     }
 
     int state_ptrs(void** p4) override {
         if (!p4) return CMCX_ERR_ARG;
-        for (int f = 0; f < 4; f++) p4[f] = cur.f[f];
+        for (int f = 0; f < 4; f++) p4[f] = cur.f[f] + G().off;
         return CMCX_OK;
     }
 
     int get_layer(void* vel, double* Tout, int ox, int oy, int oz) override {
         if (!vel || !Tout || ox < 0 || oy < 0 || oz < 0) return CMCX_ERR_ARG;
+        const Grid& Gr = G();
+        const Geom& g = Gr.g;
         if (ox == 0) ox = g.nx;
         if (oy == 0) oy = g.ny;
         if (oz == 0) oz = g.nz;
         // Solver3D::GetLayer: next->Clear(grid, NODE_OUT, 99999, ...) then FilterToArrays
         T miss = (T)99999.0f;
-        launch_fill_typed<T>(g, d_type, NODE_OUT, next, miss, miss, miss, miss, st);
+        launch_fill_typed<T>(g, d_type + Gr.off, NODE_OUT, shifted(next, Gr.off), miss, miss, miss, miss, st);
         const int64_t N = g.N;
         std::vector<T> h[4];
         for (int f = 0; f < 4; f++) {
             h[f].resize(N);
-            CMCX_HIP(hipMemcpyAsync(h[f].data(), next.f[f], sizeof(T) * N, hipMemcpyDeviceToHost, st));
+            CMCX_HIP(hipMemcpyAsync(h[f].data(), next.f[f] + Gr.off, sizeof(T) * N,
+                                    hipMemcpyDeviceToHost, st));
         }
         CMCX_HIP(hipStreamSynchronize(st));
         T* V = (T*)vel;
@@ -522,9 +696,26 @@ class Solver final : public ISolver {
     int set_boundary(const uint8_t* bv, const uint8_t* bt, const double* vx, const double* vy,
                      const double* vz, const double* Tt) override {
         if (!bv || !bt || !vx || !vy || !vz || !Tt) return CMCX_ERR_ARG;
-        upload_boundary(bv, bt, vx, vy, vz, Tt);
+        Grid& Gr = G();
+        const int64_t N = Gr.g.N, o = Gr.off;
+        for (int64_t i = 0; i < N; i++)
+            if (bv[i] > 1 || bt[i] > 1) {
+                set_error("BC kind out of range at node " + std::to_string(i));
+                return CMCX_ERR_ARG;
+            }
+        std::memcpy(&h_bcv[o], bv, N);
+        std::memcpy(&h_bct[o], bt, N);
+        CMCX_HIP(hipMemcpyAsync(d_bcv + o, bv, N, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMemcpyAsync(d_bct + o, bt, N, hipMemcpyHostToDevice, st));
+        std::vector<T> tmp(N);
+        const double* src[4] = {vx, vy, vz, Tt};
+        for (int f = 0; f < 4; f++) {
+            for (int64_t i = 0; i < N; i++) tmp[i] = (T)src[f][i];
+            CMCX_HIP(hipMemcpyAsync(d_g[f] + o, tmp.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+            CMCX_HIP(hipStreamSynchronize(st));  // tmp is reused
+        }
         if (built) {
-            free_shared = shared_free_axes(d_type, d_bcv, d_bct, g, st);
+            Gr.free_shared = shared_free_axes(&h_type[o], &h_bcv[o], &h_bct[o], Gr.g);
             prepare_path_buffers();
         }
         return CMCX_OK;
@@ -547,8 +738,8 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
-    // Times every strided-tile variant of the X and Y sweeps on this domain
-    // (scratch layers only: the state is untouched) and keeps the fastest.
+    // Times every variant of the fused X, Y and Z sweeps on these grids (scratch
+    // layers only: the state is untouched) and keeps the fastest.
     int autotune(int reps, int* choice) override {
         if (!built) return CMCX_ERR_STATE;
         if (reps < 1) reps = 3;
@@ -559,16 +750,14 @@ class Solver final : public ISolver {
             CMCX_HIP(hipEventCreate(&e1));
             for (int a = 0; a < 3; a++) {
                 if (plain_axis(a)) continue;
-                SweepParams<T> P = sweep_params(a, dt);
                 float best = 1e30f;
                 int bestv = variant[a];
                 for (int v = 0; v < kFusedVariants; v++) {
                     const int lv = a == AX_Z ? 0 : v;  // the Z work list does not depend on v
-                    ensure_lines(a, lv);
+                    ensure_work(a, lv);
                     auto launch = [&]() {
-                        launch_fused_sweep<T>(g, a, P, d_type, d_bcv, d_bct, d_g, cur, cur, half, tA,
-                                              1, lines[a][lv], std::max<int64_t>(nlines[a][lv], 0),
-                                              v, st);
+                        launch_fused_sweep<T>(d_desc, (T)3 / dt, a, work[a][lv], d_type, d_bcv, d_bct,
+                                              d_g, cur, cur, half, tA, 1, v, st);
                     };
                     launch();  // warm-up
                     CMCX_HIP(hipEventRecord(e0, st));
@@ -587,6 +776,7 @@ class Solver final : public ISolver {
             CMCX_HIP(hipEventDestroy(e0));
             CMCX_HIP(hipEventDestroy(e1));
             tune_cache()[tune_key()] = {variant[0], variant[1], variant[2]};
+            clear_graphs();
         }
         if (choice)
             for (int a = 0; a < 3; a++) choice[a] = variant[a];
@@ -597,18 +787,36 @@ class Solver final : public ISolver {
         if (axis < 0 || axis > 2 || v < 0 || v >= kFusedVariants) return CMCX_ERR_ARG;
         variant[axis] = v;
         if (built) prepare_path_buffers();
+        clear_graphs();
+        return CMCX_OK;
+    }
+
+    int set_graph(int enable) override {
+        graph_mode = enable != 0;
+        if (!graph_mode) clear_graphs();
+        return CMCX_OK;
+    }
+
+    int batch_size() const override { return (int)grids.size(); }
+
+    int select(int index) override {
+        if (index < 0 || index >= (int)grids.size()) return CMCX_ERR_ARG;
+        sel = index;
         return CMCX_OK;
     }
 
     std::string tune_key() const {
-        return std::to_string(g.nx) + "x" + std::to_string(g.ny) + "x" + std::to_string(g.nz) + "/" +
-               std::to_string(sizeof(T));
+        std::string k;
+        for (const Grid& Gr : grids)
+            k += std::to_string(Gr.g.nx) + "x" + std::to_string(Gr.g.ny) + "x" + std::to_string(Gr.g.nz) + ";";
+        return k + "/" + std::to_string(sizeof(T));
     }
 
     int step_bytes(int G, int L, double* bytes) override {
         if (!bytes) return CMCX_ERR_ARG;
         if (!built) return CMCX_ERR_STATE;
-        double S = (double)segs[0].sum_size + segs[1].sum_size + segs[2].sum_size;
+        double S = 0;
+        for (const Grid& Gr : grids) S += (double)Gr.segs[0].sum_size + Gr.segs[1].sum_size + Gr.segs[2].sum_size;
         *bytes = (double)G * L * (16.0 * sizeof(T) + 1.0) * S;
         return CMCX_OK;
     }
@@ -670,35 +878,48 @@ void cmcx_params_dimensional(double vis, double rho, double R, double k, double 
     out->t_phi = vis / (rho * cv);
 }
 
-int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, void* stream,
-                cmcx_solver** out) {
-    if (!dom || !params || !out) return CMCX_ERR_ARG;
+static int create_solver(const cmcx_domain* doms, const cmcx_params* params, int count, int dtype,
+                         void* stream, cmcx_solver** out) {
+    if (!doms || !params || !out || count < 1) return CMCX_ERR_ARG;
     *out = nullptr;
     if (dtype != CMCX_F32 && dtype != CMCX_F64) return CMCX_ERR_ARG;
-    for (int a = 0; a < 3; a++)
-        if (dom->dims[a] < 3) {
-            set_error("every dimension must be >= 3");
+    for (int i = 0; i < count; i++) {
+        const cmcx_domain* dom = &doms[i];
+        for (int a = 0; a < 3; a++)
+            if (dom->dims[a] < 3) {
+                set_error("every dimension must be >= 3");
+                return CMCX_ERR_ARG;
+            }
+        if (!dom->type || !dom->bc_vel || !dom->bc_temp || !dom->v[0] || !dom->v[1] ||
+            !dom->v[2] || !dom->T)
             return CMCX_ERR_ARG;
+        int64_t N = (int64_t)dom->dims[0] * dom->dims[1] * dom->dims[2];
+        if (N > ((int64_t)1 << 31) - 1) {
+            set_error("grid too large (node count must fit in int32)");
+            return CMCX_ERR_TOO_LARGE;
         }
-    if (!dom->type || !dom->bc_vel || !dom->bc_temp || !dom->v[0] || !dom->v[1] || !dom->v[2] ||
-        !dom->T)
-        return CMCX_ERR_ARG;
-    int64_t N = (int64_t)dom->dims[0] * dom->dims[1] * dom->dims[2];
-    if (N > ((int64_t)1 << 31) - 1) {
-        set_error("grid too large (node count must fit in int32)");
-        return CMCX_ERR_TOO_LARGE;
+        for (int64_t k = 0; k < N; k++)
+            if (dom->type[k] > 3 || dom->bc_vel[k] > 1 || dom->bc_temp[k] > 1) {
+                set_error("node type or BC kind out of range at node " + std::to_string(k) +
+                          " of grid " + std::to_string(i));
+                return CMCX_ERR_ARG;
+            }
     }
-    for (int64_t i = 0; i < N; i++)
-        if (dom->type[i] > 3 || dom->bc_vel[i] > 1 || dom->bc_temp[i] > 1) {
-            set_error("node type or BC kind out of range at node " + std::to_string(i));
-            return CMCX_ERR_ARG;
-        }
     CMCX_GUARD({
         auto* s = new cmcx_solver;
         s->dtype = dtype;
         hipStream_t st = (hipStream_t)stream;
-        if (dtype == CMCX_F32) s->impl.reset(new Solver<float>(dom, params, st));
-        else s->impl.reset(new Solver<double>(dom, params, st));
+        try {
+            if (dtype == CMCX_F32) s->impl.reset(new Solver<float>(doms, params, count, st));
+            else s->impl.reset(new Solver<double>(doms, params, count, st));
+        } catch (const std::string& e) {
+            delete s;
+            set_error(e);
+            return CMCX_ERR_TOO_LARGE;
+        } catch (...) {
+            delete s;
+            throw;
+        }
         int rc = dtype == CMCX_F32 ? static_cast<Solver<float>*>(s->impl.get())->check_mask()
                                    : static_cast<Solver<double>*>(s->impl.get())->check_mask();
         if (rc != CMCX_OK) {
@@ -708,6 +929,16 @@ int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, vo
         *out = s;
         return CMCX_OK;
     })
+}
+
+int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, void* stream,
+                cmcx_solver** out) {
+    return create_solver(dom, params, 1, dtype, stream, out);
+}
+
+int cmcx_create_batch(const cmcx_domain* doms, const cmcx_params* params, int count, int dtype,
+                      void* stream, cmcx_solver** out) {
+    return create_solver(doms, params, count, dtype, stream, out);
 }
 
 int cmcx_destroy(cmcx_solver* s) {
@@ -731,17 +962,7 @@ int cmcx_step(cmcx_solver* s, double dt, int G, int L, int compute_err, double* 
     SOLVER_CALL(s, step(dt, G, L, compute_err, err));
 }
 int cmcx_run(cmcx_solver* s, double dt, int steps, int G, int L, double* err) {
-    if (!s || !s->impl) return CMCX_ERR_ARG;
-    CMCX_GUARD({
-        int rc = CMCX_OK;
-        for (int i = 0; i < steps; i++) {
-            rc = s->impl->update_boundaries();
-            if (rc != CMCX_OK) return rc;
-            rc = s->impl->step(dt, G, L, (i % 10 == 0) || (i == steps - 1), err);
-            if (rc != CMCX_OK) return rc;
-        }
-        return rc;
-    })
+    SOLVER_CALL(s, run(dt, steps, G, L, err));
 }
 int cmcx_div_error(cmcx_solver* s, double* err) { SOLVER_CALL(s, div_error(err)); }
 int cmcx_get_fields(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T) {
@@ -770,6 +991,14 @@ int cmcx_autotune(cmcx_solver* s, int reps, int* choice) { SOLVER_CALL(s, autotu
 int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant) {
     SOLVER_CALL(s, set_tile_variant(axis, variant));
 }
+int cmcx_set_graph(cmcx_solver* s, int enable) { SOLVER_CALL(s, set_graph(enable)); }
+int cmcx_batch_size(cmcx_solver* s, int* count) {
+    if (!s || !s->impl || !count) return CMCX_ERR_ARG;
+    *count = s->impl->batch_size();
+    return CMCX_OK;
+}
+int cmcx_select(cmcx_solver* s, int index) { SOLVER_CALL(s, select(index)); }
+int cmcx_grid_errors(cmcx_solver* s, double* errs) { SOLVER_CALL(s, grid_errors(errs)); }
 int cmcx_synchronize(cmcx_solver* s) { SOLVER_CALL(s, synchronize()); }
 
 }  // extern "C"

@@ -82,29 +82,50 @@ void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const Swee
                         Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
                         int var0, int var1, T* scratch, hipStream_t st);
 
-// Fused sweep (kernels_fused.hip).  `variant` selects, for the strided (X, Y)
-// sweeps, the tile width (0 = 128-byte rows, 1 = 64-byte rows; strided_lpg gives
-// its z lines) and, for the Z sweep, the register cap (0 = compiler's choice,
-// 1 = capped for more waves per SIMD).
+// Fused sweep (kernels_fused.hip).  A launch covers a batch of independent
+// grids stored back to back in the flat arrays; `variant` selects, for the
+// strided (X, Y) sweeps, the tile width (0 = 128-byte rows, 1 = 64-byte rows;
+// strided_lpg gives its z lines) and, for the Z sweep, the register cap (0 =
+// compiler's choice, 1 = capped for more waves per SIMD).
 constexpr int kFusedVariants = 2;
+constexpr int kZWaves = 4;  // wavefronts per Z-sweep work-group (two lines each)
+
+// Descriptor of one grid of a batch.  three_dt is passed per launch.
+template <class T>
+struct InstDesc {
+    Geom g;
+    int64_t off;            // flat index of node (0, 0, 0)
+    SweepParams<T> p[3];    // per sweep axis
+};
+
+// Work-group list of one fused sweep over all grids of a batch.
+struct FusedWork {
+    int64_t nblocks = 0;
+    int32_t* d_items = nullptr;  // X/Y: {grid, tile} per work-group; Z: 2*kZWaves line ids
+                                 // (x*ny + y, -1 = padding) per work-group
+    int32_t* d_inst = nullptr;   // Z: grid of each work-group
+    int maxn = 0;                // longest line along the axis over the grids
+    int vec_ok = 1;              // Z: every line starts on a 16-byte boundary
+    void release();
+};
+
 template <class T>
 bool fused_supported(const Geom& g);
 template <class T>
 int strided_lpg(int variant);
 template <class T>
-void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
-                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
-                        Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-                        const int32_t* lines, int64_t nlines, int variant, hipStream_t st);
-// Active lines of an axis (lines holding at least one segment), line ids in
-// the natural order of the fused kernels.
-// lpg: z lines per tile of the strided (X, Y) sweeps.
-void build_active_lines(const uint8_t* d_type, const Geom& g, int axis, int lpg, int32_t** d_lines,
-                        int64_t* nlines, hipStream_t st);
+void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
+                        const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
+                        const T* const* gval, Layer4<T> in, Layer4<T> tin, Layer4<T> out,
+                        Layer4<T> tout, int merges, int variant, hipStream_t st);
+// Builds the work of `axis` from the host copy of the flat node types; lpg: z
+// lines per tile of the strided (X, Y) sweeps; elem: sizeof the field type.
+void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
+                      const std::vector<int64_t>& offs, int axis, int lpg, size_t elem,
+                      FusedWork& w);
 // Axes whose lines hold a BC_FREE node with NODE_IN on both sides (a free node
-// shared by two segments): bit a set for axis a.
-int shared_free_axes(const uint8_t* d_type, const uint8_t* d_bcv, const uint8_t* d_bct,
-                     const Geom& g, hipStream_t st);
+// shared by two segments): bit a set for axis a.  Host arrays of one grid.
+int shared_free_axes(const uint8_t* type, const uint8_t* bcv, const uint8_t* bct, const Geom& g);
 
 // Field kernels (kernels_fields.hip)
 template <class T>

@@ -33,6 +33,9 @@
 //     cache line) and all rows; lane group g of wave w owns chunk w*(64/LPG)+g.
 //   contiguous lines (Z sweep): 32 lanes per line, two lines per wavefront,
 //     lane l owns rows [l*M, l*M+M) (vector loads).
+// A launch covers a batch of independent grids stored back to back in the flat
+// field arrays: each work-group reads its grid's geometry and coefficients from
+// a descriptor table (one grid is a batch of one).
 // Work-groups are mapped to tiles XCD by XCD (block b runs on XCD b % 8), so
 // neighbouring tiles -- which share the cross-line stencil reads -- run on the
 // same XCD at about the same time and share its L2.
@@ -385,10 +388,11 @@ __device__ __forceinline__ void solve_reduced_lines(T* red, RedLayout L, int nl,
 // (waves per SIMD), so that two half-width work-groups can share a CU.
 template <class T, int AXIS, int M, int LPG, int MAXT, int MINW>
 __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
-    Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
-    const uint8_t* __restrict__ bct, const T* __restrict__ gv0, const T* __restrict__ gv1,
-    const T* __restrict__ gv2, const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin,
-    Layer4<T> out, Layer4<T> tout, int merges, const int32_t* __restrict__ tiles, int ztiles) {
+    const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
+    const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
+    const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
+    Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
+    const int32_t* __restrict__ work) {
     extern __shared__ char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);
     const T* gv[4] = {gv0, gv1, gv2, gv3};
@@ -400,14 +404,21 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     const int chunk = wave * GPW + lane / LPG;
     const int P = nw * GPW;
     const RedLayout L{2 * P, kRedFields * 2 * P + 1};
+
+    // work item: (instance, tile = outer * ztiles + z tile) of this work-group
+    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int inst = work[2 * b], tile = work[2 * b + 1];
+    const Geom g = insts[inst].g;
+    SweepParams<T> p = insts[inst].p[AXIS];
+    p.three_dt = three_dt;
     const int n = AXIS == AX_X ? g.nx : g.ny;
     const int64_t s = AXIS == AX_X ? g.sx : g.sy;
-
-    const int tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const int ztiles = (g.nz + LPG - 1) / LPG;
     const int outer = tile / ztiles;
     const int z = (tile % ztiles) * LPG + gl;
     const bool zok = z < g.nz;
-    const int64_t base = (AXIS == AX_X ? (int64_t)outer * g.sy : (int64_t)outer * g.sx) + z;
+    const int64_t base =
+        insts[inst].off + (AXIS == AX_X ? (int64_t)outer * g.sy : (int64_t)outer * g.sx) + z;
     const int r0 = chunk * M;
 
     int role[M];
@@ -496,26 +507,32 @@ __device__ __forceinline__ void store_run(T* __restrict__ p, const T* r, bool ve
 // the along-line neighbours of a chunk come from the adjacent lanes by shuffle
 // and the cross-line derivatives of w from vector loads of the neighbour lines.
 template <class T, int M, int MINW>
-__global__ void __launch_bounds__(256, MINW) fused_z_kernel(
-    Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
-    const uint8_t* __restrict__ bct, const T* __restrict__ gv0, const T* __restrict__ gv1,
-    const T* __restrict__ gv2, const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin,
-    Layer4<T> out, Layer4<T> tout, int merges, const int32_t* __restrict__ lines, int nlines,
-    int vec_ok) {
+__global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
+    const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
+    const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
+    const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
+    Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
+    const int32_t* __restrict__ lines, const int32_t* __restrict__ winst, int vec_ok) {
     extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
+    // every line of a work-group belongs to one instance
+    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int inst = winst[b];
+    const Geom g = insts[inst].g;
+    SweepParams<T> p = insts[inst].p[AX_Z];
+    p.three_dt = three_dt;
     const int n = g.nz;
     const int used = (n + M - 1) / M;  // lanes of a line holding rows (<= 32)
     const RedLayout L{2 * used, kRedFields * 2 * used + 1};
     T* red = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * 2 * L.pitch;
 
     const int sub = lane >> 5, ll = lane & 31;
-    const int li = (int)xcd_remap(blockIdx.x, gridDim.x) * (2 * nw) + 2 * wave + sub;
-    const bool live = li < nlines;
-    const int64_t base = live ? (int64_t)lines[li] * g.nz : 0;  // (x*ny + y) * nz
+    const int lid = lines[b * (2 * nw) + 2 * wave + sub];  // x*ny + y, or -1 (padding)
+    const bool live = lid >= 0;
+    const int64_t base = live ? insts[inst].off + (int64_t)lid * g.nz : 0;
     const int r0 = ll * M;
     const bool ok = live && ll < used;
     const int valid = ok ? min(M, n - r0) : 0;
@@ -683,26 +700,23 @@ bool fused_supported(const Geom& g) {
 }
 
 template <class T>
-void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
-                        const uint8_t* bcv, const uint8_t* bct, const T* const* gval,
-                        Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-                        const int32_t* lines, int64_t nlines, int variant, hipStream_t st) {
-    if (nlines == 0) return;
+void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
+                        const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
+                        const T* const* gval, Layer4<T> in, Layer4<T> tin, Layer4<T> out,
+                        Layer4<T> tout, int merges, int variant, hipStream_t st) {
+    if (w.nblocks == 0) return;
+    const unsigned blocks = (unsigned)w.nblocks;
     if (axis == AX_Z) {
-        const int waves = 4;
-        const int m = z_chunk(g.nz);
-        const int used = (g.nz + m - 1) / m;
+        const int m = z_chunk(w.maxn);
+        const int used = (w.maxn + m - 1) / m;
         const int pitch = kRedFields * 2 * used + 1;
-        size_t lds = (size_t)waves * 2 * pitch * sizeof(T);
-        unsigned blocks = (unsigned)((nlines + 2 * waves - 1) / (2 * waves));
-        // 16-byte vectors need every line to start on a 16-byte boundary
-        const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
+        size_t lds = (size_t)kZWaves * 2 * pitch * sizeof(T);
         // variant 1 caps the registers for 4 (FP32) / 2 (FP64) waves per SIMD
         constexpr int MW = sizeof(T) == 8 ? 2 : 4;
-#define LZ(MM, W)                                                                                \
-    hipLaunchKernelGGL((fused_z_kernel<T, MM, W>), dim3(blocks), dim3(64 * waves), lds, st, g, \
-                       p, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, tin, out,     \
-                       tout, merges, lines, (int)nlines, vec_ok)
+#define LZ(MM, W)                                                                               \
+    hipLaunchKernelGGL((fused_z_kernel<T, MM, W>), dim3(blocks), dim3(64 * kZWaves), lds, st,  \
+                       insts, three_dt, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, \
+                       tin, out, tout, merges, w.d_items, w.d_inst, w.vec_ok)
         if (variant == 0) {
             if (m == 2) LZ(2, 1);
             else if (m == 4) LZ(4, 1);
@@ -716,20 +730,17 @@ void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const 
         }
 #undef LZ
     } else {
-        const int n = axis == AX_X ? g.nx : g.ny;
         constexpr int LW = variant_lpg<T>(0), LN = variant_lpg<T>(1);
         const int lpg = variant ? LN : LW;
-        const int waves = strided_waves_for(n, lpg);
+        const int waves = strided_waves_for(w.maxn, lpg);
         const int P = waves * (64 / lpg);
         const int pitch = kRedFields * 2 * P + 1;
         size_t lds = (size_t)lpg * pitch * sizeof(T);
-        const int ztiles = (g.nz + lpg - 1) / lpg;
 #define LS(AX, LPGV)                                                                             \
     hipLaunchKernelGGL((fused_strided_kernel<T, AX, kChunk, LPGV, 32 * LPGV,                      \
                                              (LPGV == LN ? LPGV / 4 : LPGV / 8)>),                \
-                       dim3((unsigned)nlines), dim3(64 * waves), lds, st, g, p, type, bcv, bct,   \
-                       gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, lines,     \
-                       ztiles)
+                       dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, type, bcv, bct,  \
+                       gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, w.d_items)
         if (axis == AX_X) {
             if (variant) LS(AX_X, LN);
             else LS(AX_X, LW);
@@ -742,52 +753,75 @@ void launch_fused_sweep(const Geom& g, int axis, const SweepParams<T>& p, const 
     CMCX_LAUNCH_CHECK();
 }
 
-// Active work items of an axis: Z -> lines x*ny + y holding a NODE_IN node;
-// X/Y -> tiles outer*ztiles + zt whose lpg lines hold a NODE_IN node.
-void build_active_lines(const uint8_t* d_type, const Geom& g, int axis, int lpg, int32_t** d_lines,
-                        int64_t* nlines, hipStream_t st) {
-    std::vector<uint8_t> t(g.N);
-    CMCX_HIP(hipMemcpyAsync(t.data(), d_type, g.N, hipMemcpyDeviceToHost, st));
-    CMCX_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> items;
-    if (axis == AX_Z) {
-        for (int x = 0; x < g.nx; x++)
-            for (int y = 0; y < g.ny; y++) {
-                const uint8_t* row = &t[x * g.sx + y * g.sy];
-                bool any = false;
-                for (int z = 0; z < g.nz && !any; z++) any = row[z] == NODE_IN;
-                if (any) items.push_back(x * g.ny + y);
-            }
-    } else {
-        const int ztiles = (g.nz + lpg - 1) / lpg;
-        const int nouter = axis == AX_X ? g.ny : g.nx;
-        const int n = axis == AX_X ? g.nx : g.ny;
-        const int64_t so = axis == AX_X ? g.sy : g.sx, sr = axis == AX_X ? g.sx : g.sy;
-        for (int o = 0; o < nouter; o++)
-            for (int zt = 0; zt < ztiles; zt++) {
-                bool any = false;
-                for (int r = 0; r < n && !any; r++)
-                    for (int z = zt * lpg; z < std::min(g.nz, (zt + 1) * lpg) && !any; z++)
-                        any = t[o * so + r * sr + z] == NODE_IN;
-                if (any) items.push_back(o * ztiles + zt);
-            }
+void FusedWork::release() {
+    hipFree(d_items);
+    hipFree(d_inst);
+    d_items = d_inst = nullptr;
+    nblocks = 0;
+}
+
+// Work of one axis over all instances.  Z: lines x*ny + y holding a NODE_IN
+// node, each instance's list padded with -1 to whole work-groups (2 lines per
+// wave), plus the instance of every work-group.  X/Y: (instance, tile) pairs,
+// tile = outer * ztiles + z tile, for the tiles of lpg z lines holding a
+// NODE_IN node.
+void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
+                      const std::vector<int64_t>& offs, int axis, int lpg, size_t elem,
+                      FusedWork& w) {
+    w.release();
+    std::vector<int32_t> items, winst;
+    w.maxn = 0;
+    w.vec_ok = 1;
+    for (size_t i = 0; i < geoms.size(); i++) {
+        const Geom& g = geoms[i];
+        const uint8_t* t = h_type + offs[i];
+        if (axis == AX_Z) {
+            w.maxn = std::max(w.maxn, g.nz);
+            if (((size_t)g.nz * elem) % 16 || ((size_t)offs[i] * elem) % 16) w.vec_ok = 0;
+            size_t first = items.size();
+            for (int x = 0; x < g.nx; x++)
+                for (int y = 0; y < g.ny; y++) {
+                    const uint8_t* row = &t[x * g.sx + y * g.sy];
+                    bool any = false;
+                    for (int z = 0; z < g.nz && !any; z++) any = row[z] == NODE_IN;
+                    if (any) items.push_back(x * g.ny + y);
+                }
+            const size_t per = 2 * kZWaves;
+            while ((items.size() - first) % per) items.push_back(-1);
+            for (size_t k = first; k < items.size(); k += per) winst.push_back((int32_t)i);
+        } else {
+            const int n = axis == AX_X ? g.nx : g.ny;
+            w.maxn = std::max(w.maxn, n);
+            const int ztiles = (g.nz + lpg - 1) / lpg;
+            const int nouter = axis == AX_X ? g.ny : g.nx;
+            const int64_t so = axis == AX_X ? g.sy : g.sx, sr = axis == AX_X ? g.sx : g.sy;
+            for (int o = 0; o < nouter; o++)
+                for (int zt = 0; zt < ztiles; zt++) {
+                    bool any = false;
+                    for (int r = 0; r < n && !any; r++)
+                        for (int z = zt * lpg; z < std::min(g.nz, (zt + 1) * lpg) && !any; z++)
+                            any = t[o * so + r * sr + z] == NODE_IN;
+                    if (any) {
+                        items.push_back((int32_t)i);
+                        items.push_back(o * ztiles + zt);
+                    }
+                }
+        }
     }
-    *nlines = (int64_t)items.size();
-    *d_lines = nullptr;
+    w.nblocks = axis == AX_Z ? (int64_t)winst.size() : (int64_t)items.size() / 2;
     if (!items.empty()) {
-        CMCX_HIP(hipMalloc((void**)d_lines, sizeof(int32_t) * items.size()));
-        CMCX_HIP(hipMemcpy(*d_lines, items.data(), sizeof(int32_t) * items.size(),
+        CMCX_HIP(hipMalloc((void**)&w.d_items, sizeof(int32_t) * items.size()));
+        CMCX_HIP(hipMemcpy(w.d_items, items.data(), sizeof(int32_t) * items.size(),
+                           hipMemcpyHostToDevice));
+    }
+    if (!winst.empty()) {
+        CMCX_HIP(hipMalloc((void**)&w.d_inst, sizeof(int32_t) * winst.size()));
+        CMCX_HIP(hipMemcpy(w.d_inst, winst.data(), sizeof(int32_t) * winst.size(),
                            hipMemcpyHostToDevice));
     }
 }
 
-int shared_free_axes(const uint8_t* d_type, const uint8_t* d_bcv, const uint8_t* d_bct,
-                     const Geom& g, hipStream_t st) {
-    std::vector<uint8_t> t(g.N), bv(g.N), bt(g.N);
-    CMCX_HIP(hipMemcpyAsync(t.data(), d_type, g.N, hipMemcpyDeviceToHost, st));
-    CMCX_HIP(hipMemcpyAsync(bv.data(), d_bcv, g.N, hipMemcpyDeviceToHost, st));
-    CMCX_HIP(hipMemcpyAsync(bt.data(), d_bct, g.N, hipMemcpyDeviceToHost, st));
-    CMCX_HIP(hipStreamSynchronize(st));
+int shared_free_axes(const uint8_t* t, const uint8_t* bv, const uint8_t* bt, const Geom& g) {
     int mask = 0;
     const int64_t str[3] = {g.sx, g.sy, 1};
     const int dim[3] = {g.nx, g.ny, g.nz};
@@ -809,14 +843,13 @@ template bool fused_supported<float>(const Geom&);
 template bool fused_supported<double>(const Geom&);
 template int strided_lpg<float>(int);
 template int strided_lpg<double>(int);
-template void launch_fused_sweep<float>(const Geom&, int, const SweepParams<float>&, const uint8_t*,
-                                        const uint8_t*, const uint8_t*, const float* const*,
-                                        Layer4<float>, Layer4<float>, Layer4<float>, Layer4<float>,
-                                        int, const int32_t*, int64_t, int, hipStream_t);
-template void launch_fused_sweep<double>(const Geom&, int, const SweepParams<double>&,
+template void launch_fused_sweep<float>(const InstDesc<float>*, float, int, const FusedWork&,
+                                        const uint8_t*, const uint8_t*, const uint8_t*,
+                                        const float* const*, Layer4<float>, Layer4<float>,
+                                        Layer4<float>, Layer4<float>, int, int, hipStream_t);
+template void launch_fused_sweep<double>(const InstDesc<double>*, double, int, const FusedWork&,
                                          const uint8_t*, const uint8_t*, const uint8_t*,
                                          const double* const*, Layer4<double>, Layer4<double>,
-                                         Layer4<double>, Layer4<double>, int, const int32_t*,
-                                         int64_t, int, hipStream_t);
+                                         Layer4<double>, Layer4<double>, int, int, hipStream_t);
 
 }  // namespace cmcx

@@ -110,6 +110,19 @@ void cmcx_params_dimensional(double vis, double rho, double R, double k, double 
    stream: a hipStream_t, or NULL for the default stream. */
 int cmcx_create(const cmcx_domain* dom, const cmcx_params* params, int dtype, void* stream,
                 cmcx_solver** out);
+/* Several independent grids (each with its own mask, boundary data and
+   parameters) advanced together: every fused sweep covers all grids in one
+   launch.  The per-grid calls below (segments, fields, layer output, boundary
+   data, div_error) address the grid chosen with cmcx_select (0 at creation);
+   cmcx_step / cmcx_run advance all grids, and the divergence guard fires when
+   any grid exceeds the threshold (no grid is advanced then).  No reference
+   counterpart (the reference runs one grid per process). */
+int cmcx_create_batch(const cmcx_domain* doms, const cmcx_params* params, int count, int dtype,
+                      void* stream, cmcx_solver** out);
+int cmcx_batch_size(cmcx_solver* s, int* count);
+int cmcx_select(cmcx_solver* s, int index);
+/* Divergence errors of every grid from the last evaluation (count doubles). */
+int cmcx_grid_errors(cmcx_solver* s, double* errs);
 int cmcx_destroy(cmcx_solver* s);
 
 int cmcx_set_options(cmcx_solver* s, int path, int merge_order);
@@ -141,11 +154,18 @@ int cmcx_step(cmcx_solver* s, double dt, int num_global, int num_local, int comp
    FluidSolver3D.cpp:226-242): error evaluated every 10th step and on the last. */
 int cmcx_run(cmcx_solver* s, double dt, int steps, int num_global, int num_local, double* err);
 
-/* Divergence error of the current state. */
+/* Replays each time step as a HIP graph (captured on first use per dt,
+   num_global, num_local and state buffer; cmcx_run's graph includes the
+   boundary fill).  Options, variants and segments invalidate the graphs.
+   0 = direct launches.  No reference counterpart. */
+int cmcx_set_graph(cmcx_solver* s, int enable);
+
+/* Divergence error of the current state (selected grid). */
 int cmcx_div_error(cmcx_solver* s, double* err);
 
 /* Current state to host arrays of the field type (float for CMCX_F32, double for
-   CMCX_F64), N elements each; any pointer may be NULL.  p = v_T * T. */
+   CMCX_F64), N elements each; any pointer may be NULL.  p = v_T * T.
+   Selected grid of a batch. */
 int cmcx_get_fields(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T);
 /* Same into device buffers (N elements each), ordered on the handle's stream. */
 int cmcx_get_fields_device(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T);
