@@ -86,15 +86,16 @@ def run_plain_steps(dom, path: int, steps: int, warmup: int, dtype: str = "f32")
     s.close()
 
 
-def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32", tune: bool = True):
-    from cmcx.solver import make_solver
-
-    s = make_solver(dom, dtype, path=path)
-    G, L = dom.num_global, dom.num_local
-    variants = s.autotune(3) if (tune and path == 1) else None
+def time_solver(s, cells: int, dt: float, G: int, L: int, steps: int, warmup: int,
+                tune: bool = True, graph: bool = False):
+    """Times `steps` steps (boundary fill + TimeStep) of a built solver: one pass
+    with HIP events around every sweep launch (kernel time), one wall-clock pass
+    without events (direct launches or graph replays)."""
+    variants = s.autotune(3) if tune else None
+    s.set_graph(graph)
     for _ in range(warmup):
         s.UpdateBoundaries()
-        s.TimeStep(dom.dt, G, L, False)
+        s.TimeStep(dt, G, L, False)
     s.synchronize()
     _sync()
     # kernel timing pass (HIP events around each sweep launch on the solver's stream)
@@ -102,32 +103,66 @@ def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32", tune:
     s.profile_read(reset=True)
     for _ in range(steps):
         s.UpdateBoundaries()
-        s.TimeStep(dom.dt, G, L, False)
+        s.TimeStep(dt, G, L, False)
     sweep_ms, launches = s.profile_read(reset=True)
     s.profile(False)
     s.synchronize()
     _sync()
     # wall-clock pass without events
     t0 = time.perf_counter()
-    for i in range(steps):
-        s.UpdateBoundaries()
-        s.TimeStep(dom.dt, G, L, i == steps - 1)
+    if graph:
+        s.run(steps, dt, G, L)  # graph of boundary fill + step, error every 10th step
+    else:
+        for i in range(steps):
+            s.UpdateBoundaries()
+            s.TimeStep(dt, G, L, i % 10 == 0 or i == steps - 1)
     s.synchronize()
     _sync()
     wall = time.perf_counter() - t0
     step_bytes = s.step_bytes(G, L)
-    err = s.div_error()
-    s.close()
     return {
         "ms_per_step": wall / steps * 1e3,
-        "cell_updates_per_s": dom.n * steps / wall,
+        "cell_updates_per_s": cells * steps / wall,
         "sweep_ms_per_launch": sweep_ms / max(launches, 1),
         "sweep_launches": launches,
         "sweep_bytes_per_step": step_bytes,
         "sweep_achieved_gbs": step_bytes * steps / (sweep_ms * 1e-3) / 1e9 if sweep_ms > 0 else 0.0,
-        "div_error": err,
         "variants": variants,
     }
+
+
+def time_path(dom, path: int, steps: int, warmup: int, dtype: str = "f32", tune: bool = True,
+              graph: bool = False):
+    from cmcx.solver import make_solver
+
+    s = make_solver(dom, dtype, path=path)
+    r = time_solver(s, dom.n, dom.dt, dom.num_global, dom.num_local, steps, warmup,
+                    tune=tune and path == 1, graph=graph)
+    r["div_error"] = s.div_error()
+    s.close()
+    return r
+
+
+def time_batch(steps: int, warmup: int, count: int = 64):
+    """The multi-grid workload: `count` seeded boxes (edges from {32,48,64,96})
+    advanced together; fused (one launch per sweep, graph replay) against the
+    plain per-segment kernels (one launch per grid and sweep)."""
+    from cmcx import domains
+    from cmcx.solver import make_batch
+
+    doms = domains.batch_boxes(count=count, seed=0)
+    cells = sum(d.n for d in doms)
+    d0 = doms[0]
+    out = {"grids": count, "cells": cells, "seed": 0}
+    s = make_batch(doms, "f32", path=1)
+    out["fused"] = time_solver(s, cells, d0.dt, d0.num_global, 1, steps, warmup, tune=True,
+                               graph=True)
+    s.close()
+    s = make_batch(doms, "f32", path=0)
+    out["plain"] = time_solver(s, cells, d0.dt, d0.num_global, 1, max(2, steps // 4), 1,
+                               tune=False, graph=False)
+    s.close()
+    return out
 
 
 def cpu_baseline(dom, budget_s: float = 20.0):
@@ -167,6 +202,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="only run the steps (no events, no autotuning, no JSON line)")
@@ -202,11 +238,15 @@ def main():
                         args.steps, args.warmup)
         print(f"pmc-mode: {args.steps} steps of the {head_path} path done", flush=True)
         return
-    head = time_path(dom, CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN,
-                     args.steps, args.warmup)
+    head_id = CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN
+    direct = time_path(dom, head_id, args.steps, args.warmup)
+    graphed = time_path(dom, head_id, args.steps, args.warmup, tune=False, graph=True)
+    head = graphed if graphed["cell_updates_per_s"] >= direct["cell_updates_per_s"] else direct
+    head["variants"] = direct["variants"]
     plain = None
     if head_path == "fused" and not args.no_plain:
         plain = time_path(dom, CMCX_PATH_PLAIN, max(2, args.steps // 4), 1)
+    batch = None if args.no_batch else time_batch(max(4, args.steps // 2), 2)
     cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
 
     achieved = head["sweep_achieved_gbs"]
@@ -235,6 +275,7 @@ def main():
             "num_local": dom.num_local,
             "path": head_path,
             "tile_variants": head["variants"],
+            "graph": head is graphed,
             "parallelism": "single-gpu",
         },
         "roofline": {
@@ -250,6 +291,24 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    line["step_launch"] = {
+        "direct_ms_per_step": direct["ms_per_step"],
+        "graph_ms_per_step": graphed["ms_per_step"],
+    }
+    if batch is not None:
+        bf, bp = batch["fused"], batch["plain"]
+        line["batch"] = {
+            "workload": f"{batch['grids']} boxes, edges from {{32,48,64,96}}, seed {batch['seed']}, fp32",
+            "cells": batch["cells"],
+            "value": bf["cell_updates_per_s"],
+            "unit": "cell-updates/s",
+            "ms_per_step": bf["ms_per_step"],
+            "sweep_achieved_gbs": bf["sweep_achieved_gbs"],
+            "hbm_frac": bf["sweep_achieved_gbs"] / HBM_PEAK_GBS,
+            "tile_variants": bf["variants"],
+            "plain_value": bp["cell_updates_per_s"],
+            "speedup_over_plain": bf["cell_updates_per_s"] / bp["cell_updates_per_s"],
+        }
     if plain is not None:
         line["plain"] = {
             "value": plain["cell_updates_per_s"],

@@ -461,9 +461,244 @@ struct Solver {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Discrete adjoint of TimeStep (CPU merge order, num_local = 1): the transpose
+// of every linear step of the forward map, applied in reverse.  Per sweep and
+// segment the forward solves M(temp) x = d(in, temp); its adjoint solves
+// M^T lam = xbar (+ the merge's share of tempbar) and scatters
+//   inbar = (3/dt) lam,  Mbar = -lam x^T  -> tempbar (advecting velocity) and
+//   the viscosity gradients, dbar -> tempbar through the pressure and
+//   dissipation stencils, and lam on Dirichlet end rows -> node-value gradients.
+// The Picard layer is merged as temp <- (temp + x)/2 (twice after X), so its
+// adjoint scales tempbar by 1/2 (1/4) and feeds 1/2 (3/4) into xbar.
+// Node values enter through UpdateBoundaries (state and Picard layer on
+// BOUND/VALVE nodes) and the Dirichlet end rows.  Shared BC_FREE end nodes
+// are not supported (the forward's write order there is arbitrary).
+template <class T>
+struct Adjoint {
+    Solver<T>& s;
+    size_t N;
+    int G = 0;
+    T dt = 0;
+    struct Tape {
+        std::vector<T> S;         // state after UpdateBoundaries, [4][N]
+        std::vector<T> tin, out;  // per sweep k = 3*it + {0: Z, 1: Y, 2: X}: [3G][4][N]
+    };
+    std::vector<Tape> tape;
+    Layer<T> lt;                  // Picard layer of the sweep being reversed (with halo)
+
+    explicit Adjoint(Solver<T>& sv) : s(sv), N(sv.N) { lt.init(s.nx, s.ny, s.nz, s.dx, s.dy, s.dz); }
+
+    void save(Layer<T>* L, std::vector<T>& dst, size_t slot) {
+        for (int f = 0; f < 4; f++)
+            std::memcpy(&dst[(slot * 4 + f) * N], &L->f[f][L->halo], sizeof(T) * N);
+    }
+
+    // TimeStep (merge mode 0, L = 1) keeping the layers the adjoint needs.
+    void record_step(T dt_, int G_) {
+        dt = dt_;
+        G = G_;
+        Tape tp;
+        tp.S.resize(4 * N);
+        tp.tin.resize((size_t)3 * G * 4 * N);
+        tp.out.resize((size_t)3 * G * 4 * N);
+        save(s.cur, tp.S, 0);
+        s.copy_typed(s.cur, s.next, NODE_BOUND);
+        s.copy_typed(s.cur, s.next, NODE_VALVE);
+        for (int v = 0; v < 4; v++) s.temp->f[v] = s.cur->f[v];
+        for (int it = 0; it < G; it++) {
+            Layer<T>* ins[3] = {s.cur, s.next, s.half};
+            Layer<T>* outs[3] = {s.next, s.half, s.next};
+            const int dirs[3] = {AX_Z, AX_Y, AX_X};
+            for (int q = 0; q < 3; q++) {
+                size_t k = (size_t)3 * it + q;
+                save(s.temp, tp.tin, k);
+                s.sweep(dirs[q], dt, ins[q], s.temp, outs[q], 0, 4);
+                s.merge(outs[q], s.temp);
+                save(outs[q], tp.out, k);
+            }
+            s.merge(s.next, s.temp);
+        }
+        std::swap(s.cur, s.next);
+        tape.push_back(std::move(tp));
+    }
+
+    inline bool in_node(size_t i) const { return s.type[i] == NODE_IN; }
+
+    // Reverses one sweep.  xb: adjoint of the sweep output (NODE_IN nodes);
+    // tb: adjoint of the Picard layer after the sweep, replaced by the adjoint
+    // before it; inb receives the adjoint of the sweep input.
+    void adj_sweep(int dir, const T* tinL, const T* outL, const std::vector<T>& xb,
+                   std::vector<T>& tb, T alpha, T beta, std::vector<T>& inb, std::vector<T>& gb,
+                   double* pb) {
+        for (int f = 0; f < 4; f++) std::memcpy(&lt.f[f][lt.halo], tinL + f * N, sizeof(T) * N);
+        std::vector<Seg>& list = s.segs[dir];
+        std::vector<T> lam(4 * N, (T)0);
+        const int mx = s.max_n + 2;
+        std::vector<T> a(mx), b(mx), c(mx), d(mx), A(mx), B(mx), C(mx), R(mx), X(mx);
+        for (const Seg& sg : list) {
+            const int n = sg.size;
+            for (int var = 0; var < 4; var++) {
+                std::fill(a.begin(), a.end(), (T)0);
+                std::fill(c.begin(), c.end(), (T)0);
+                s.apply_bc0(sg.px, sg.py, sg.pz, var, b[0], c[0], d[0]);
+                s.apply_bc1(sg.ex, sg.ey, sg.ez, var, a[n - 1], b[n - 1], d[n - 1]);
+                s.build_matrix(dt, sg.px, sg.py, sg.pz, var, dir, a.data(), b.data(), c.data(),
+                               d.data(), n, &lt, &lt);
+                for (int t = 0; t < n; t++) {
+                    A[t] = t > 0 ? c[t - 1] : (T)0;
+                    B[t] = b[t];
+                    C[t] = t < n - 1 ? a[t + 1] : (T)0;
+                    int i, j, k;
+                    s.seg_cell(sg, dir, t, i, j, k);
+                    size_t id = s.idx(i, j, k);
+                    R[t] = in_node(id) ? xb[var * N + id] + beta * tb[var * N + id] : (T)0;
+                }
+                solve_tridiagonal(A.data(), B.data(), C.data(), R.data(), X.data(), n);
+                for (int t = 0; t < n; t++) {
+                    int i, j, k;
+                    s.seg_cell(sg, dir, t, i, j, k);
+                    size_t id = s.idx(i, j, k);
+                    if (t > 0 && t < n - 1) {
+                        lam[var * N + id] = X[t];
+                    } else {
+                        // end rows: x = node value unless BC_FREE
+                        bool fr = var == VAR_T ? s.bc_temp[id] == BC_FREE : s.bc_vel[id] == BC_FREE;
+                        if (!fr) gb[var * N + id] += X[t];
+                    }
+                }
+            }
+        }
+        for (size_t i = 0; i < N; i++)
+            if (in_node(i))
+                for (int f = 0; f < 4; f++) tb[f * N + i] *= alpha;
+        const size_t st[3] = {(size_t)s.ny * s.nz, (size_t)s.nz, 1};
+        const T hh[3] = {s.dx, s.dy, s.dz};
+        const size_t sd = st[dir];
+        const T h = hh[dir];
+        const int dv = dir;  // velocity component along the sweep
+        for (const Seg& sg : list)
+            for (int t = 1; t < sg.size - 1; t++) {
+                int i, j, k;
+                s.seg_cell(sg, dir, t, i, j, k);
+                const size_t p = s.idx(i, j, k);
+                T lm[4], xm[4], x0[4], xp[4];
+                for (int f = 0; f < 4; f++) {
+                    lm[f] = lam[f * N + p];
+                    xm[f] = outL[f * N + p - sd];
+                    x0[f] = outL[f * N + p];
+                    xp[f] = outL[f * N + p + sd];
+                    inb[f * N + p] = lm[f] * 3 / dt;
+                }
+                // matrix adjoints per matrix (velocity fields share one)
+                T abv = 0, bbv = 0, cbv = 0;
+                for (int f = 0; f < 3; f++) {
+                    abv -= lm[f] * xm[f];
+                    bbv -= lm[f] * x0[f];
+                    cbv -= lm[f] * xp[f];
+                }
+                T abt = -lm[3] * xm[3], bbt = -lm[3] * x0[3], cbt = -lm[3] * xp[3];
+                tb[dv * N + p] += (-abv + cbv - abt + cbt) / (2 * h);
+                pb[1] += (double)((-abv - cbv + 2 * bbv) / (h * h));
+                pb[2] += (double)((-abt - cbt + 2 * bbt) / (h * h));
+                // pressure term of the velocity along the sweep
+                const T* tT = tinL + 3 * N;
+                const T gT = (tT[p + sd] - tT[p - sd]) / (2 * h);
+                pb[0] += (double)(-lm[dv] * gT);
+                tb[3 * N + p + sd] += -s.v_T * lm[dv] / (2 * h);
+                tb[3 * N + p - sd] += s.v_T * lm[dv] / (2 * h);
+                // dissipation of the T row
+                const T* tu = tinL;
+                const T* tv = tinL + N;
+                const T* tw = tinL + 2 * N;
+                auto D = [&](const T* q, int ax) {
+                    return (q[p + st[ax]] - q[p - st[ax]]) / (2 * hh[ax]);
+                };
+                auto put = [&](int f, int ax, T val) {
+                    tb[f * N + p + st[ax]] += val / (2 * hh[ax]);
+                    tb[f * N + p - st[ax]] -= val / (2 * hh[ax]);
+                };
+                const T kk = s.t_phi * lm[3];
+                T phi;
+                if (dir == AX_X) {
+                    T u_x = D(tu, 0), v_x = D(tv, 0), w_x = D(tw, 0), u_y = D(tu, 1), u_z = D(tu, 2);
+                    phi = 2 * u_x * u_x + v_x * v_x + w_x * w_x + v_x * u_y + w_x * u_z;
+                    put(0, 0, kk * 4 * u_x);
+                    put(1, 0, kk * (2 * v_x + u_y));
+                    put(2, 0, kk * (2 * w_x + u_z));
+                    put(0, 1, kk * v_x);
+                    put(0, 2, kk * w_x);
+                } else if (dir == AX_Y) {
+                    T u_y = D(tu, 1), v_y = D(tv, 1), w_y = D(tw, 1), v_x = D(tv, 0), v_z = D(tv, 2);
+                    phi = u_y * u_y + 2 * v_y * v_y + w_y * w_y + u_y * v_x + w_y * v_z;
+                    put(0, 1, kk * (2 * u_y + v_x));
+                    put(1, 1, kk * 4 * v_y);
+                    put(2, 1, kk * (2 * w_y + v_z));
+                    put(1, 0, kk * u_y);
+                    put(1, 2, kk * w_y);
+                } else {
+                    T u_z = D(tu, 2), v_z = D(tv, 2), w_z = D(tw, 2), w_x = D(tw, 0), w_y = D(tw, 1);
+                    phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * w_x + v_z * w_y;
+                    put(0, 2, kk * (2 * u_z + w_x));
+                    put(1, 2, kk * (2 * v_z + w_y));
+                    put(2, 2, kk * 4 * w_z);
+                    put(2, 0, kk * u_z);
+                    put(2, 1, kk * v_z);
+                }
+                pb[3] += (double)(lm[3] * phi);
+            }
+    }
+
+    // Reverses one recorded step: cb holds the adjoint of the step's result on
+    // NODE_IN nodes and receives the adjoint of its start state (NODE_IN).
+    void adj_step(const Tape& tp, std::vector<T>& cb, std::vector<T>& gb, double* pb) {
+        std::vector<T> xb(4 * N, (T)0), tb(4 * N, (T)0), cacc(4 * N, (T)0), inb(4 * N, (T)0);
+        for (size_t i = 0; i < N; i++)
+            if (in_node(i))
+                for (int f = 0; f < 4; f++) xb[f * N + i] = cb[f * N + i];
+        for (int it = G - 1; it >= 0; it--)
+            for (int q = 2; q >= 0; q--) {
+                const int dir = q == 2 ? AX_X : q == 1 ? AX_Y : AX_Z;
+                const size_t k = (size_t)3 * it + q;
+                const T alpha = q == 2 ? (T)0.25 : (T)0.5, beta = q == 2 ? (T)0.75 : (T)0.5;
+                std::fill(inb.begin(), inb.end(), (T)0);
+                adj_sweep(dir, &tp.tin[k * 4 * N], &tp.out[k * 4 * N], xb, tb, alpha, beta, inb, gb, pb);
+                if (q == 0) {
+                    for (size_t i = 0; i < 4 * N; i++) cacc[i] += inb[i];
+                    std::fill(xb.begin(), xb.end(), (T)0);
+                } else {
+                    xb = inb;
+                }
+            }
+        // temp <- cur at the step start; UpdateBoundaries wrote the node values
+        // into the state on BOUND/VALVE nodes.
+        for (size_t i = 0; i < N; i++)
+            for (int f = 0; f < 4; f++) {
+                if (in_node(i)) cb[f * N + i] = cacc[f * N + i] + tb[f * N + i];
+                else {
+                    cb[f * N + i] = 0;
+                    if (s.type[i] == NODE_BOUND || s.type[i] == NODE_VALVE) gb[f * N + i] += tb[f * N + i];
+                }
+            }
+    }
+
+    void backward(const double* seed, double* gbar, double* cbar0, double* pbar) {
+        std::vector<T> cb(4 * N, (T)0), gb(4 * N, (T)0);
+        for (size_t i = 0; i < 4 * N; i++) cb[i] = in_node(i % N) ? (T)seed[i] : (T)0;
+        double pb[4] = {0, 0, 0, 0};
+        for (int n = (int)tape.size() - 1; n >= 0; n--) adj_step(tape[n], cb, gb, pb);
+        for (size_t i = 0; i < 4 * N; i++) {
+            if (gbar) gbar[i] = (double)gb[i];
+            if (cbar0) cbar0[i] = (double)cb[i];
+        }
+        for (int q = 0; q < 4; q++) pbar[q] = pb[q];
+    }
+};
+
 struct Handle {
     int dtype;
     void* s;
+    void* adj = nullptr;
 };
 
 }  // namespace orc
@@ -507,6 +742,10 @@ void* orc_create(int dtype, int nx, int ny, int nz, double dx, double dy, double
 }
 
 void orc_destroy(void* h) {
+    if (((Handle*)h)->adj) {
+        if (((Handle*)h)->dtype == 0) delete (Adjoint<float>*)((Handle*)h)->adj;
+        else delete (Adjoint<double>*)((Handle*)h)->adj;
+    }
     DISPATCH(h, delete s);
     delete (Handle*)h;
 }
@@ -586,6 +825,33 @@ void orc_get_layer(void* h, float* vel_xyz, double* outT, int ox, int oy, int oz
                     vel_xyz[3 * ind + 2] = (float)L->e(VAR_W, x, y, z);
                     outT[ind] = (double)L->e(VAR_T, x, y, z);
                 }
+    });
+}
+
+// Adjoint: `steps` recorded time steps (UpdateBoundaries + TimeStep with
+// num_local = 1, CPU merge order), then the reverse pass.  seed: adjoint of
+// the final state [4][N] (u, v, w, T); gbar: gradient of the node values
+// [4][N]; cbar0: gradient of the initial state [4][N] (NODE_IN nodes); pbar:
+// gradients of (v_T, v_vis, t_vis, t_phi).
+void orc_adj_record(void* h, int steps, double dt, int G) {
+    Handle* H = (Handle*)h;
+    DISPATCH(h, {
+        using TT = decltype(s->dx);
+        if (!H->adj) H->adj = new Adjoint<TT>(*s);
+        auto* A = (Adjoint<TT>*)H->adj;
+        for (int i = 0; i < steps; i++) {
+            s->update_boundaries();
+            A->record_step((TT)dt, G);
+        }
+    });
+}
+
+void orc_adj_backward(void* h, const double* seed, double* gbar, double* cbar0, double* pbar) {
+    Handle* H = (Handle*)h;
+    DISPATCH(h, {
+        using TT = decltype(s->dx);
+        auto* A = (Adjoint<TT>*)H->adj;
+        if (A) A->backward(seed, gbar, cbar0, pbar);
     });
 }
 

@@ -62,6 +62,10 @@ def load():
     L.orc_segments_count.argtypes = [_c_u8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.orc_segments_count.restype = ctypes.c_int
     L.orc_segments.argtypes = [_c_u8, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_i32]
+    L.orc_adj_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.orc_adj_record.restype = None
+    L.orc_adj_backward.argtypes = [ctypes.c_void_p, _c_d, _c_d, _c_d, _c_d]
+    L.orc_adj_backward.restype = None
     L.orc_set_threads.argtypes = [ctypes.c_int]
     L.orc_max_threads.restype = ctypes.c_int
     # 2D
@@ -178,6 +182,21 @@ class Oracle3D:
         T = np.zeros(ox * oy * oz)
         self.L.orc_get_layer(self.h, _ptr(v, _c_f), _ptr(T, _c_d), ox, oy, oz)
         return v, T
+
+    def adj_record(self, steps, dt, G):
+        """Runs `steps` recorded steps (UpdateBoundaries + TimeStep, L = 1)."""
+        self.L.orc_adj_record(self.h, steps, dt, G)
+
+    def adj_backward(self, seed):
+        """seed: adjoint of the final state, shape (4, N).  Returns (gbar (4, N),
+        cbar0 (4, N), pbar (4,) for v_T, v_vis, t_vis, t_phi)."""
+        n = self.dom.n
+        sd = np.ascontiguousarray(seed, dtype=np.float64).reshape(4 * n)
+        gb = np.zeros(4 * n)
+        cb = np.zeros(4 * n)
+        pb = np.zeros(4)
+        self.L.orc_adj_backward(self.h, _ptr(sd, _c_d), _ptr(gb, _c_d), _ptr(cb, _c_d), _ptr(pb, _c_d))
+        return gb.reshape(4, n), cb.reshape(4, n), pb
 
     def run(self, steps, dt=None, G=None, L=None, compute_every=10):
         d = self.dom
