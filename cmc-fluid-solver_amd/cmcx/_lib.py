@@ -84,6 +84,11 @@ SIGNATURES = [
     ("cmcx_select", ctypes.c_int, [_vp, ctypes.c_int]),
     ("cmcx_grid_errors", ctypes.c_int, [_vp, _dp]),
     ("cmcx_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("cmcx_set_params", ctypes.c_int, [_vp, ctypes.POINTER(cmcx_params)]),
+    ("cmcx_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("cmcx_adjoint_backward", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                             ctypes.POINTER(_vp), _dp]),
+    ("cmcx_adjoint_release", ctypes.c_int, [_vp]),
     ("cmcx2d_create", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params), ctypes.c_int,
                                      _vp, ctypes.POINTER(_vp)]),
     ("cmcx2d_update_boundaries", ctypes.c_int, [_vp]),

@@ -198,6 +198,43 @@ class AdiSolver3D:
     def set_tile_variant(self, axis: int, variant: int):
         check(lib().cmcx_set_tile_variant(self.h, axis, variant), "cmcx_set_tile_variant")
 
+    def set_params(self, params: FluidParams):
+        prm = _lib.cmcx_params(params.v_T, params.v_vis, params.t_vis, params.t_phi)
+        check(lib().cmcx_set_params(self.h, ctypes.byref(prm)), "cmcx_set_params")
+        self.params = params
+
+    # -- reverse mode ------------------------------------------------------------------
+    def adjoint_forward(self, steps: int, dt: float, num_global: int = 4, checkpoint_every: int = 20):
+        """Rollout of `steps` steps keeping checkpoints for :meth:`adjoint_backward`."""
+        check(lib().cmcx_adjoint_forward(self.h, float(dt), int(steps), int(num_global),
+                                         int(checkpoint_every)), "cmcx_adjoint_forward")
+
+    def adjoint_backward(self, seed):
+        """seed: dL/d(final u, v, w, T), array-like (4, N) (None entries = 0).
+        Returns (gbar (4, N), sbar (4, N), pbar (4,)): gradients of the node
+        values, of the initial state and of (v_T, v_vis, t_vis, t_phi)."""
+        n = self.grid.n
+        keep = []
+        sp = (_lib._vp * 4)()
+        for f in range(4):
+            if seed is None or seed[f] is None:
+                sp[f] = None
+            else:
+                a = np.ascontiguousarray(seed[f], dtype=self.np_dtype).reshape(n)
+                keep.append(a)
+                sp[f] = a.ctypes.data
+        gb = np.zeros((4, n), self.np_dtype)
+        sb = np.zeros((4, n), self.np_dtype)
+        gp = (_lib._vp * 4)(*[gb[f].ctypes.data for f in range(4)])
+        bp = (_lib._vp * 4)(*[sb[f].ctypes.data for f in range(4)])
+        pb = np.zeros(4, np.float64)
+        check(lib().cmcx_adjoint_backward(self.h, sp, gp, bp, pb.ctypes.data_as(_lib._dp)),
+              "cmcx_adjoint_backward")
+        return gb, sb, pb
+
+    def adjoint_release(self):
+        check(lib().cmcx_adjoint_release(self.h), "cmcx_adjoint_release")
+
     def set_graph(self, enable: bool = True):
         """Replay each step as a HIP graph (captured on first use)."""
         check(lib().cmcx_set_graph(self.h, 1 if enable else 0), "cmcx_set_graph")

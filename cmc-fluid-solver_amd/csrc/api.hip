@@ -60,6 +60,11 @@ struct ISolver {
     virtual int batch_size() const = 0;
     virtual int select(int index) = 0;
     virtual int grid_errors(double* errs) = 0;
+    virtual int set_params(const cmcx_params* p) = 0;
+    virtual int adjoint_forward(double dt, int steps, int G, int every) = 0;
+    virtual int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
+                                 double* pbar) = 0;
+    virtual int adjoint_release() = 0;
     virtual int synchronize() = 0;
 };
 
@@ -117,6 +122,21 @@ class Solver final : public ISolver {
     bool graph_mode = false;
     std::map<std::tuple<int, double, int, int, const void*>, hipGraphExec_t> graphs;
     hipStream_t cap = nullptr;
+    // reverse mode over a recorded rollout
+    struct Adj {
+        bool recorded = false;
+        T dt = 0;
+        int G = 0, steps = 0, every = 0;
+        T* ck = nullptr;       // checkpoints of the state, [nck][4][NT]
+        int nck = 0;
+        T* tape = nullptr;     // per step: S, Picard layers 1..3G, sweep outputs 0..3G-1
+        int64_t tape_layers = 0;
+        T* work = nullptr;     // xbar, tbar, state-adjoint sum, lam, node-value gradient, final state
+        double* pp = nullptr;  // per-block parameter partials
+        int32_t* tiles[3] = {nullptr, nullptr, nullptr};
+        int64_t ntiles[3] = {0, 0, 0};
+        int ltiles[3] = {0, 0, 0};
+    } adj;
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -235,6 +255,11 @@ class Solver final : public ISolver {
         hipFree(d_part);
         hipFree(d_nonin);
         hipFree(d_bvlist);
+        hipFree(adj.ck);
+        hipFree(adj.tape);
+        hipFree(adj.work);
+        hipFree(adj.pp);
+        for (auto* t : adj.tiles) hipFree(t);
         for (auto& e : ev_pool) {
             hipEventDestroy(e.first);
             hipEventDestroy(e.second);
@@ -548,6 +573,16 @@ class Solver final : public ISolver {
         return ex;
     }
 
+    // (Boundary fill +) one step, replayed as a graph or launched directly.
+    void launch_step(bool ub, T dt, int G, int L) {
+        if (graph_mode && !prof) {
+            CMCX_HIP(hipGraphLaunch(step_graph(ub, dt, G, L), st));
+        } else {
+            if (ub) update_boundaries();
+            enqueue_step(dt, G, L);
+        }
+    }
+
     // Launches the step (graph or direct), then the divergence guard and swap.
     int advance(bool ub, double dt_d, int G, int L, int compute_err, double* err) {
         if (!built) {
@@ -556,12 +591,7 @@ class Solver final : public ISolver {
         }
         if (G < 0 || L < 1) return CMCX_ERR_ARG;
         const T dt = (T)dt_d;
-        if (graph_mode && !prof) {
-            CMCX_HIP(hipGraphLaunch(step_graph(ub, dt, G, L), st));
-        } else {
-            if (ub) update_boundaries();
-            enqueue_step(dt, G, L);
-        }
+        launch_step(ub, dt, G, L);
         if (compute_err) diff_error = eval_div(next);
         if (err) *err = diff_error;
         if (prof) harvest_events();
@@ -821,6 +851,179 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    int set_params(const cmcx_params* p) override {
+        if (!p) return CMCX_ERR_ARG;
+        G().prm = *p;
+        upload_desc();
+        clear_graphs();
+        return CMCX_OK;
+    }
+
+    // ---- reverse mode ----------------------------------------------------------
+    Layer4<T> lay(T* base, int64_t l) const {
+        Layer4<T> r;
+        for (int f = 0; f < 4; f++) r.f[f] = base + (l * 4 + f) * NT;
+        return r;
+    }
+
+    void copy_layer(Layer4<T> dst, Layer4<T> src) {
+        for (int f = 0; f < 4; f++)
+            CMCX_HIP(hipMemcpyAsync(dst.f[f], src.f[f], sizeof(T) * NT, hipMemcpyDeviceToDevice, st));
+    }
+
+    int adjoint_forward(double dt, int steps, int G, int every) override {
+        if (!built) return CMCX_ERR_STATE;
+        if (steps < 1 || G < 1 || every < 1) return CMCX_ERR_ARG;
+        if (grids.size() != 1 || merge_order != CMCX_MERGE_CPU) {
+            set_error("adjoint: one grid with the CPU merge order");
+            return CMCX_ERR_UNSUPPORTED;
+        }
+        if (!adjoint_supported<T>(grids[0].g) || grids[0].free_shared) {
+            set_error("adjoint: axes of at most 256 nodes and no BC_FREE node shared by two segments");
+            return CMCX_ERR_UNSUPPORTED;
+        }
+        const int nck = (steps + every - 1) / every;
+        if (nck > adj.nck) {
+            hipFree(adj.ck);
+            adj.ck = nullptr;
+            CMCX_HIP(hipMalloc((void**)&adj.ck, sizeof(T) * 4 * NT * nck));
+            adj.nck = nck;
+        }
+        const T dtt = (T)dt;
+        for (int i = 0; i < steps; i++) {
+            if (i % every == 0) copy_layer(lay(adj.ck, i / every), cur);
+            launch_step(true, dtt, G, 1);
+            std::swap(cur, next);
+        }
+        adj.recorded = true;
+        adj.dt = dtt;
+        adj.G = G;
+        adj.steps = steps;
+        adj.every = every;
+        return CMCX_OK;
+    }
+
+    // UpdateBoundaries + one step from `cur`, keeping every layer the reverse
+    // step reads in tape slot base sb; leaves the new state in cur.
+    void record_step(T* sb) {
+        const int G = adj.G;
+        const T dt = adj.dt;
+        update_boundaries();
+        Layer4<T> S = lay(sb, 0);
+        copy_layer(S, cur);
+        Layer4<T> tin = S;
+        const int axes[3] = {AX_Z, AX_Y, AX_X};
+        for (int it = 0; it < G; it++)
+            for (int q = 0; q < 3; q++) {
+                const int k = 3 * it + q;
+                Layer4<T> in = q == 0 ? S : lay(sb, 1 + 3 * G + k - 1);
+                Layer4<T> out = lay(sb, 1 + 3 * G + k), tout = lay(sb, k + 1);
+                sweep(axes[q], in, tin, out, tout, q == 2 ? 2 : 1, 0, 4, dt);
+                tin = tout;
+            }
+        copy_layer(cur, lay(sb, 1 + 3 * G + 3 * G - 1));
+    }
+
+    void reverse_step(T* sb) {
+        const int G = adj.G;
+        const Geom& g = grids[0].g;
+        Layer4<T> XB = lay(adj.work, 0), TB = lay(adj.work, 1), CB = lay(adj.work, 2),
+                  LB = lay(adj.work, 3), GB = lay(adj.work, 4);
+        const int axes[3] = {AX_Z, AX_Y, AX_X};
+        for (int it = G - 1; it >= 0; it--)
+            for (int q = 2; q >= 0; q--) {
+                const int k = 3 * it + q, a = axes[q];
+                Layer4<T> tin = lay(sb, k == 0 ? 0 : k), out = lay(sb, 1 + 3 * G + k);
+                const T alpha = q == 2 ? (T)0.25 : (T)0.5, beta = q == 2 ? (T)0.75 : (T)0.5;
+                launch_adj_sweep<T>(g, a, sweep_params(0, a, adj.dt), d_type, d_bcv, d_bct, tin, out,
+                                    XB, TB, CB, LB, GB, alpha, beta, q == 0 ? 1 : 0, adj.tiles[a],
+                                    adj.ntiles[a], adj.ltiles[a], adj.pp, st);
+            }
+        launch_adj_step_end<T>(NT, d_type, TB, CB, XB, GB, st);
+    }
+
+    int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
+                         double* pbar) override {
+        if (!adj.recorded) {
+            set_error("adjoint_backward before adjoint_forward");
+            return CMCX_ERR_STATE;
+        }
+        const int G = adj.G;
+        const Geom& g = grids[0].g;
+        const int64_t per = 1 + 6 * (int64_t)G;
+        const int64_t need = per * std::min(adj.every, adj.steps);
+        if (need > adj.tape_layers) {
+            hipFree(adj.tape);
+            adj.tape = nullptr;
+            CMCX_HIP(hipMalloc((void**)&adj.tape, sizeof(T) * 4 * NT * need));
+            adj.tape_layers = need;
+        }
+        if (!adj.work) {
+            CMCX_HIP(hipMalloc((void**)&adj.work, sizeof(T) * 4 * NT * 6));
+            CMCX_HIP(hipMalloc((void**)&adj.pp, sizeof(double) * 4 * adj_point_blocks()));
+            for (int a = 0; a < 3; a++)
+                build_adj_tiles(h_type.data(), g, a, adj_lpg<T>(), &adj.tiles[a], &adj.ntiles[a],
+                                &adj.ltiles[a]);
+        }
+        Layer4<T> XB = lay(adj.work, 0), GB = lay(adj.work, 4), FIN = lay(adj.work, 5);
+        copy_layer(FIN, cur);
+        CMCX_HIP(hipMemsetAsync(adj.work, 0, sizeof(T) * 4 * NT * 5, st));
+        CMCX_HIP(hipMemsetAsync(adj.pp, 0, sizeof(double) * 4 * adj_point_blocks(), st));
+        if (seed)
+            for (int f = 0; f < 4; f++)
+                if (seed[f])
+                    CMCX_HIP(hipMemcpyAsync(XB.f[f], seed[f], sizeof(T) * g.N, hipMemcpyHostToDevice, st));
+        // Picard-layer slots: node values on non-IN nodes, as the forward's
+        // temp holds the boundary-filled state there (sweeps write NODE_IN only)
+        Layer4<T> gl;
+        for (int f = 0; f < 4; f++) gl.f[f] = d_g[f];
+        const int64_t nsteps_tape = std::min(adj.every, adj.steps);
+        for (int64_t i = 0; i < nsteps_tape; i++)
+            for (int k = 1; k <= 3 * G; k += 2) {
+                Layer4<T> a = lay(adj.tape + i * per * 4 * NT, k);
+                Layer4<T> b = k + 1 <= 3 * G ? lay(adj.tape + i * per * 4 * NT, k + 1) : a;
+                launch_prep_step_list<T>(n_nonin, d_nonin, d_type, gl, a, a, b, st);
+            }
+        const int nseg = (adj.steps + adj.every - 1) / adj.every;
+        for (int sgm = nseg - 1; sgm >= 0; sgm--) {
+            const int s0 = sgm * adj.every;
+            const int k = std::min(adj.every, adj.steps - s0);
+            copy_layer(cur, lay(adj.ck, sgm));
+            for (int i = 0; i < k; i++) record_step(adj.tape + i * per * 4 * NT);
+            for (int i = k - 1; i >= 0; i--) reverse_step(adj.tape + i * per * 4 * NT);
+        }
+        if (gbar)
+            for (int f = 0; f < 4; f++)
+                if (gbar[f])
+                    CMCX_HIP(hipMemcpyAsync(gbar[f], GB.f[f], sizeof(T) * g.N, hipMemcpyDeviceToHost, st));
+        if (sbar)
+            for (int f = 0; f < 4; f++)
+                if (sbar[f])
+                    CMCX_HIP(hipMemcpyAsync(sbar[f], XB.f[f], sizeof(T) * g.N, hipMemcpyDeviceToHost, st));
+        std::vector<double> pp(4 * adj_point_blocks());
+        CMCX_HIP(hipMemcpyAsync(pp.data(), adj.pp, sizeof(double) * pp.size(), hipMemcpyDeviceToHost, st));
+        copy_layer(cur, FIN);
+        CMCX_HIP(hipStreamSynchronize(st));
+        if (pbar) {
+            for (int q = 0; q < 4; q++) pbar[q] = 0;
+            for (int b = 0; b < adj_point_blocks(); b++)
+                for (int q = 0; q < 4; q++) pbar[q] += pp[4 * b + q];
+        }
+        return CMCX_OK;
+    }
+
+    int adjoint_release() override {
+        hipStreamSynchronize(st);
+        hipFree(adj.ck);
+        hipFree(adj.tape);
+        adj.ck = nullptr;
+        adj.tape = nullptr;
+        adj.nck = 0;
+        adj.tape_layers = 0;
+        adj.recorded = false;
+        return CMCX_OK;
+    }
+
     int synchronize() override {
         CMCX_HIP(hipStreamSynchronize(st));
         return CMCX_OK;
@@ -999,6 +1202,15 @@ int cmcx_batch_size(cmcx_solver* s, int* count) {
 }
 int cmcx_select(cmcx_solver* s, int index) { SOLVER_CALL(s, select(index)); }
 int cmcx_grid_errors(cmcx_solver* s, double* errs) { SOLVER_CALL(s, grid_errors(errs)); }
+int cmcx_set_params(cmcx_solver* s, const cmcx_params* p) { SOLVER_CALL(s, set_params(p)); }
+int cmcx_adjoint_forward(cmcx_solver* s, double dt, int steps, int num_global, int checkpoint_every) {
+    SOLVER_CALL(s, adjoint_forward(dt, steps, num_global, checkpoint_every));
+}
+int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* gbar,
+                          void* const* sbar, double* pbar) {
+    SOLVER_CALL(s, adjoint_backward(seed, gbar, sbar, pbar));
+}
+int cmcx_adjoint_release(cmcx_solver* s) { SOLVER_CALL(s, adjoint_release()); }
 int cmcx_synchronize(cmcx_solver* s) { SOLVER_CALL(s, synchronize()); }
 
 }  // extern "C"

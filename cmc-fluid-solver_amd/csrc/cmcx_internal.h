@@ -127,6 +127,31 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
 // shared by two segments): bit a set for axis a.  Host arrays of one grid.
 int shared_free_axes(const uint8_t* type, const uint8_t* bcv, const uint8_t* bct, const Geom& g);
 
+// Adjoint sweeps (kernels_adjoint.hip).  Lines per tile of the transposed line
+// solve: 128 bytes of lines (adj_lpg); every axis at most 256 nodes long.
+template <class T>
+int adj_lpg();
+template <class T>
+bool adjoint_supported(const Geom& g);
+// Tiles (outer * ltiles + line tile) of `axis` holding a NODE_IN node, from host types.
+void build_adj_tiles(const uint8_t* h_type, const Geom& g, int axis, int lpg, int32_t** d_tiles,
+                     int64_t* ntiles, int* ltiles);
+// Number of per-block parameter partial slots (4 doubles each) of launch_adj_sweep.
+int adj_point_blocks();
+// Reverses one sweep: xb/tb (adjoint of the sweep output / of the Picard layer
+// after it) -> lam, tb (before it), xb (adjoint of the sweep input) or, with
+// zmode, cacc += input adjoint and xb = 0; gb += lam on Dirichlet end rows;
+// pp[4*b..] += parameter partials (v_T, v_vis, t_vis, t_phi).
+template <class T>
+void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
+                      const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
+                      Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc, Layer4<T> lam, Layer4<T> gb,
+                      T alpha, T beta, int zmode, const int32_t* tiles, int64_t ntiles, int ltiles,
+                      double* pp, hipStream_t st);
+template <class T>
+void launch_adj_step_end(int64_t N, const uint8_t* type, Layer4<T> tb, Layer4<T> cacc,
+                         Layer4<T> xb, Layer4<T> gb, hipStream_t st);
+
 // Field kernels (kernels_fields.hip)
 template <class T>
 void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const* gval,

@@ -1,0 +1,232 @@
+// Device building blocks of the line-wide tridiagonal solves (forward sweeps
+// and their adjoint): row roles, the chunked partition elimination and the
+// parallel cyclic reduction of the reduced systems staged in LDS.
+//
+// Chunked partition method (the Thomas/PCR hybrid of Kim, Wu, Chang and Hwu,
+// "A scalable tridiagonal solver for GPUs", 2011): the rows of a line are cut
+// into chunks of M consecutive rows held by one lane.  A downward then upward
+// elimination inside the chunk expresses every inner row through the chunk's
+// first and last unknowns, x_j = E_j + F_j x_first + G_j x_last; the first and
+// last rows of all chunks form a tridiagonal "reduced" system of 2 rows per
+// chunk, staged through LDS and solved by parallel cyclic reduction with one
+// reduced row per lane and cross-lane shuffles; each lane then reconstructs
+// its inner rows.  Two matrices share the work: matrix 0 (velocity) serves
+// right-hand sides 0..2, matrix 1 (T) right-hand side 3.
+#pragma once
+
+#include "cmcx_internal.h"
+
+namespace cmcx {
+namespace lsolve {
+
+enum { RL_NONE = 0, RL_IN = 1, RL_START = 2, RL_END = 3, RL_SHARED = 4 };
+constexpr int kRedFields = 10;  // A_v B_v C_v A_t B_t C_t D_u D_v D_w D_t
+constexpr uint8_t kOutside = 255;
+
+__device__ __forceinline__ int row_role(uint8_t t, uint8_t tl, uint8_t tr) {
+    if (t == NODE_IN) return RL_IN;
+    bool L = tl == NODE_IN, R = tr == NODE_IN;
+    if (L && R) return RL_SHARED;
+    if (R) return RL_START;
+    if (L) return RL_END;
+    return RL_NONE;
+}
+
+// Boundary or identity row of one matrix (ApplyBC0/ApplyBC1); uses_val: the
+// RHS is the node value.
+template <class T>
+__device__ __forceinline__ void bc_row(int role, bool free_bc, T& a, T& b, T& c, bool& uses_val) {
+    uses_val = false;
+    if (role == RL_START && free_bc) { a = 0; b = 2; c = -1; return; }
+    if (role == RL_END && free_bc) { a = -1; b = 2; c = 0; return; }
+    a = 0; b = 1; c = 0;
+    uses_val = role != RL_NONE;
+}
+
+inline __device__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    const unsigned X = 8;
+    unsigned q = nb / X, rem = nb % X;
+    unsigned xcd = b % X, local = b / X;
+    return xcd < rem ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
+}
+
+template <class T>
+struct Row {
+    T av, bv, cv, at, bt, ct;
+    T d[4];
+};
+
+// Chunk state of one lane: inner rows 1..M-2 as x_j = E_j + F_j x_0 + G_j x_{M-1}
+// (F, G per matrix, E per field).
+template <class T, int M>
+struct Chunk {
+    T Fv[M], Gv[M], Ft[M], Gt[M];
+    T E[4][M];
+};
+
+// Downward elimination, row j in 1..M-2: x_j = p_j + q_j x_0 - r_j x_{j+1},
+// (p, q, r) kept in (E, F, G).
+template <class T, int M>
+__device__ __forceinline__ void chunk_down(Chunk<T, M>& ch, int j, const Row<T>& r) {
+    if (j == 1) {
+        T iv = 1 / r.bv, it = 1 / r.bt;
+        ch.Fv[j] = -r.av * iv; ch.Gv[j] = r.cv * iv;
+        ch.Ft[j] = -r.at * it; ch.Gt[j] = r.ct * it;
+#pragma unroll
+        for (int v = 0; v < 3; v++) ch.E[v][j] = r.d[v] * iv;
+        ch.E[3][j] = r.d[3] * it;
+    } else {
+        T iv = 1 / (r.bv - r.av * ch.Gv[j - 1]);
+        T it = 1 / (r.bt - r.at * ch.Gt[j - 1]);
+#pragma unroll
+        for (int v = 0; v < 3; v++) ch.E[v][j] = (r.d[v] - r.av * ch.E[v][j - 1]) * iv;
+        ch.E[3][j] = (r.d[3] - r.at * ch.E[3][j - 1]) * it;
+        ch.Fv[j] = -r.av * ch.Fv[j - 1] * iv;
+        ch.Ft[j] = -r.at * ch.Ft[j - 1] * it;
+        ch.Gv[j] = r.cv * iv;
+        ch.Gt[j] = r.ct * it;
+    }
+}
+
+// Upward elimination: (p, q, r) -> (E, F, G).
+template <class T, int M>
+__device__ __forceinline__ void chunk_up(Chunk<T, M>& ch) {
+    ch.Gv[M - 2] = -ch.Gv[M - 2];
+    ch.Gt[M - 2] = -ch.Gt[M - 2];
+#pragma unroll
+    for (int j = M - 3; j >= 1; j--) {
+        T rv = ch.Gv[j], rt = ch.Gt[j];
+#pragma unroll
+        for (int v = 0; v < 3; v++) ch.E[v][j] = ch.E[v][j] - rv * ch.E[v][j + 1];
+        ch.E[3][j] = ch.E[3][j] - rt * ch.E[3][j + 1];
+        ch.Fv[j] = ch.Fv[j] - rv * ch.Fv[j + 1];
+        ch.Ft[j] = ch.Ft[j] - rt * ch.Ft[j + 1];
+        ch.Gv[j] = -rv * ch.Gv[j + 1];
+        ch.Gt[j] = -rt * ch.Gt[j + 1];
+    }
+}
+
+// Reduced rows of a chunk: f couples (x_{-1}, x_0, x_{M-1}), l couples
+// (x_0, x_{M-1}, x_M).  Fields A_v B_v C_v A_t B_t C_t D_u D_v D_w D_t.
+template <class T, int M>
+__device__ __forceinline__ void chunk_reduced(const Chunk<T, M>& ch, const Row<T>& r0,
+                                              const Row<T>& rl, T* f, T* l) {
+    if (M >= 3) {
+        f[0] = r0.av; f[1] = r0.bv + r0.cv * ch.Fv[1]; f[2] = r0.cv * ch.Gv[1];
+        f[3] = r0.at; f[4] = r0.bt + r0.ct * ch.Ft[1]; f[5] = r0.ct * ch.Gt[1];
+#pragma unroll
+        for (int v = 0; v < 3; v++) f[6 + v] = r0.d[v] - r0.cv * ch.E[v][1];
+        f[9] = r0.d[3] - r0.ct * ch.E[3][1];
+        l[0] = rl.av * ch.Fv[M - 2]; l[1] = rl.bv + rl.av * ch.Gv[M - 2]; l[2] = rl.cv;
+        l[3] = rl.at * ch.Ft[M - 2]; l[4] = rl.bt + rl.at * ch.Gt[M - 2]; l[5] = rl.ct;
+#pragma unroll
+        for (int v = 0; v < 3; v++) l[6 + v] = rl.d[v] - rl.av * ch.E[v][M - 2];
+        l[9] = rl.d[3] - rl.at * ch.E[3][M - 2];
+    } else {
+        f[0] = r0.av; f[1] = r0.bv; f[2] = r0.cv; f[3] = r0.at; f[4] = r0.bt; f[5] = r0.ct;
+        l[0] = rl.av; l[1] = rl.bv; l[2] = rl.cv; l[3] = rl.at; l[4] = rl.bt; l[5] = rl.ct;
+#pragma unroll
+        for (int v = 0; v < 4; v++) { f[6 + v] = r0.d[v]; l[6 + v] = rl.d[v]; }
+    }
+}
+
+// Parallel cyclic reduction over one reduced system per group of RP lanes:
+// the lane holds row i = lane % RP (rows >= R are identity, and every access
+// stays inside the lane's group).  On return D holds x.
+template <class T>
+__device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, int RP) {
+    // A[q], B[q], C[q]: matrix q (0 vel, 1 T); D[0..2] use matrix 0, D[3] matrix 1
+    if (i >= R) {
+        A[0] = A[1] = C[0] = C[1] = 0;
+        B[0] = B[1] = 1;
+        D[0] = D[1] = D[2] = D[3] = 0;
+    }
+    if (i == 0) A[0] = A[1] = 0;
+    if (i == R - 1) C[0] = C[1] = 0;
+    const int lane = threadIdx.x & 63;
+    for (int st = 1; st < RP; st <<= 1) {
+        const bool hm = i - st >= 0, hp = i + st < RP;
+        const int lm = hm ? lane - st : lane, lp = hp ? lane + st : lane;
+        T Am[2], Bm[2], Cm[2], Ap[2], Bp[2], Cp[2], Dm[4], Dp[4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            Am[q] = __shfl(A[q], lm, 64); Bm[q] = __shfl(B[q], lm, 64); Cm[q] = __shfl(C[q], lm, 64);
+            Ap[q] = __shfl(A[q], lp, 64); Bp[q] = __shfl(B[q], lp, 64); Cp[q] = __shfl(C[q], lp, 64);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) { Dm[v] = __shfl(D[v], lm, 64); Dp[v] = __shfl(D[v], lp, 64); }
+        if (!hm) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) { Am[q] = 0; Bm[q] = 1; Cm[q] = 0; }
+#pragma unroll
+            for (int v = 0; v < 4; v++) Dm[v] = 0;
+        }
+        if (!hp) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) { Ap[q] = 0; Bp[q] = 1; Cp[q] = 0; }
+#pragma unroll
+            for (int v = 0; v < 4; v++) Dp[v] = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            T k1 = A[q] / Bm[q];
+            T k2 = C[q] / Bp[q];
+            B[q] = B[q] - Cm[q] * k1 - Ap[q] * k2;
+            if (q == 0) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) D[v] = D[v] - Dm[v] * k1 - Dp[v] * k2;
+            } else {
+                D[3] = D[3] - Dm[3] * k1 - Dp[3] * k2;
+            }
+            A[q] = -Am[q] * k1;
+            C[q] = -Cp[q] * k2;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 3; v++) D[v] = D[v] / B[0];
+    D[3] = D[3] / B[1];
+}
+
+// Reduced systems staged in LDS as [line][field][row] with a line pitch of
+// kRedFields*R + 1 elements (conflict-free for both the chunk writes, one lane
+// per line, and the PCR reads, one lane per row).
+struct RedLayout {
+    int R, pitch;
+    __device__ __forceinline__ int at(int line, int field, int row) const {
+        return line * pitch + field * R + row;
+    }
+};
+
+// Solves `nl` reduced systems held in LDS with the lanes of `nw` wavefronts
+// (this one is wave `w`); writes x over the D fields.
+template <class T>
+__device__ __forceinline__ void solve_reduced_lines(T* red, RedLayout L, int nl, int w, int nw) {
+    const int lane = threadIdx.x & 63;
+    int RP = 1;
+    while (RP < L.R) RP <<= 1;
+    const int lpp = 64 / RP;                 // lines per pass of one wave
+    const int i = lane % RP;
+    for (int base = w * lpp; base < nl; base += nw * lpp) {
+        const int line = base + lane / RP;
+        const int Rl = line < nl ? L.R : 0;  // rows of this lane's system (0: no system)
+        const bool act = i < Rl;
+        T A[2], B[2], C[2], D[4];
+        if (act) {
+            A[0] = red[L.at(line, 0, i)]; B[0] = red[L.at(line, 1, i)]; C[0] = red[L.at(line, 2, i)];
+            A[1] = red[L.at(line, 3, i)]; B[1] = red[L.at(line, 4, i)]; C[1] = red[L.at(line, 5, i)];
+#pragma unroll
+            for (int v = 0; v < 4; v++) D[v] = red[L.at(line, 6 + v, i)];
+        } else {
+            A[0] = A[1] = C[0] = C[1] = 0; B[0] = B[1] = 1;
+            D[0] = D[1] = D[2] = D[3] = 0;
+        }
+        pcr_solve<T>(A, B, C, D, i, Rl, RP);
+        if (act) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[L.at(line, 6 + v, i)] = D[v];
+        }
+    }
+}
+
+}  // namespace lsolve
+}  // namespace cmcx

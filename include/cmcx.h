@@ -200,6 +200,30 @@ int cmcx_autotune(cmcx_solver* s, int reps, int* choice);
    use left to the compiler, 1 = capped for more waves per SIMD.  No reference
    counterpart (tuning knob). */
 int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant);
+/* FluidParams of the selected grid (e.g. a new viscosity between rollouts). */
+int cmcx_set_params(cmcx_solver* s, const cmcx_params* params);
+
+/* ---- reverse mode: gradients of a rollout (no reference counterpart) -------
+   cmcx_adjoint_forward runs `steps` steps (UpdateBoundaries + TimeStep with
+   num_local = 1, no divergence evaluation) from the current state, keeping a
+   device checkpoint of the state every `checkpoint_every` steps; the final
+   state becomes the current state.  cmcx_adjoint_backward then takes
+   seed[4] = dL/d(final u, v, w, T) (host arrays of the field type, N each,
+   NULL = 0; only NODE_IN nodes are read) and returns, for the recorded
+   rollout, gbar[4] = dL/d(node values v, T) (non-zero on BOUND/VALVE nodes),
+   sbar[4] = dL/d(initial state) on NODE_IN nodes and pbar[4] = dL/d(v_T, v_vis,
+   t_vis, t_phi).  Each checkpoint interval is recomputed once with every
+   sweep's layers kept on the device, then reversed step by step (transposed
+   line solves in reverse sweep order and the adjoint of the Picard merges).
+   Any array pointer may be NULL.  One grid, CPU merge order, axes of at most
+   256 nodes, no BC_FREE node shared by two segments (else
+   CMCX_ERR_UNSUPPORTED).  The current state is unchanged by the backward pass.
+   cmcx_adjoint_release frees the checkpoints and the recording. */
+int cmcx_adjoint_forward(cmcx_solver* s, double dt, int steps, int num_global, int checkpoint_every);
+int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* gbar,
+                          void* const* sbar, double* pbar);
+int cmcx_adjoint_release(cmcx_solver* s);
+
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
 int cmcx_synchronize(cmcx_solver* s);
