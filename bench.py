@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cmc-fluid-solver_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -165,6 +167,140 @@ def time_batch(steps: int, warmup: int, count: int = 64):
     return out
 
 
+def _probe(shape, size: int = 16):
+    """Probe box behind the cylinder (whose centre lies in the upstream half):
+    x in [nx/2+1, nx/2+1+size), centred in y and z."""
+    nx, ny, nz = shape
+    m = np.zeros(shape, bool)
+    x0 = nx // 2 + 1
+    m[x0:x0 + size, ny // 2 - size // 2:ny // 2 + size // 2, nz // 2 - size // 2:nz // 2 + size // 2] = True
+    return m.ravel()
+
+
+def _probe_ke(fields, mask):
+    n = mask.sum()
+    u, v, w = (np.asarray(fields[k], np.float64) for k in range(3))
+    seed = np.zeros((4, u.size))
+    for k, a in enumerate((u, v, w)):
+        seed[k][mask] = a[mask] / n
+    return 0.5 * float(np.sum((u * u + v * v + w * w)[mask])) / n, seed
+
+
+def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int = 96):
+    """The inlet-control gradient workload: 96^3 channel with a seeded cylinder,
+    loss = mean kinetic energy over a 16^3 downstream probe after 200 steps,
+    gradient w.r.t. the 32x32 inlet control and nu; FP32 forward and adjoint
+    timed (adjoint = checkpointed backward including the recompute), gradients
+    checked against central differences (eps 1e-3) of the FP64 forward along
+    `ndir` seeded directions, and against the FP64 adjoint."""
+    from cmcx import domains
+    from cmcx.solver import make_solver
+
+    d, control = domains.cylinder_channel(n=n, steps=steps)
+    G = d.num_global
+    mask = _probe(d.shape)
+    ids = np.arange(d.ny * d.nz)
+    nc = control.shape[0]
+
+    def inlet(c):
+        vx = d.vx.copy()
+        vx[ids] = np.kron(c, np.ones((3, 3)))[:d.ny, :d.nz].ravel()
+        return vx
+
+    def block_sum(g):
+        g = g[ids].reshape(d.ny, d.nz)
+        return g[:3 * nc, :3 * nc].reshape(nc, 3, nc, 3).sum(axis=(1, 3))
+
+    def grads(dtype, timed):
+        s = make_solver(d, dtype, path=1)
+        if timed:
+            s.autotune(3)
+        init = s.fields()
+        st0 = [init[0], init[1], init[2], init[4]]
+        out = {}
+        if timed:
+            for _ in range(3):  # warm-up
+                s.UpdateBoundaries()
+                s.TimeStep(d.dt, G, 1, False)
+            s.set_fields(*st0)
+            s.synchronize()
+            _sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.UpdateBoundaries()
+                s.TimeStep(d.dt, G, 1, False)
+            s.synchronize()
+            _sync()
+            out["forward_s"] = time.perf_counter() - t0
+            s.set_fields(*st0)
+        _sync()
+        t0 = time.perf_counter()
+        s.adjoint_forward(steps, d.dt, G, every)
+        s.synchronize()
+        _sync()
+        out["recorded_forward_s"] = time.perf_counter() - t0
+        loss, seed = _probe_ke(s.fields(), mask)
+        t0 = time.perf_counter()
+        gb, _, pb = s.adjoint_backward(seed)
+        _sync()
+        out["backward_s"] = time.perf_counter() - t0
+        out["loss"] = loss
+        out["gc"] = block_sum(gb[0].astype(np.float64))
+        out["gnu"] = float(pb[1])
+        s.adjoint_release()
+
+        def L(c, nu):
+            p = d.params()
+            p.v_vis = nu
+            s.set_params(p)
+            s.set_boundary(d.bc_vel, d.bc_temp, inlet(c), d.vy, d.vz, d.T)
+            s.set_fields(*st0)
+            for _ in range(steps):
+                s.UpdateBoundaries()
+                s.TimeStep(d.dt, G, 1, False)
+            return _probe_ke(s.fields(), mask)[0]
+
+        out["L"] = L
+        out["solver"] = s
+        return out
+
+    r32 = grads("f32", True)
+    r64 = grads("f64", False)
+    nu0 = d.params().v_vis
+    rng = np.random.default_rng(0)
+    eps = 1e-3
+    e32, e64 = [float("nan")], [float("nan")]
+    if ndir:
+        e32, e64 = [], []
+    for _ in range(ndir):
+        dc = rng.normal(size=control.shape)
+        dn = rng.normal() * nu0
+        fd = (r64["L"](control + eps * dc, nu0 + eps * dn)
+              - r64["L"](control - eps * dc, nu0 - eps * dn)) / (2 * eps)
+        a64 = float(np.sum(r64["gc"] * dc) + r64["gnu"] * dn)
+        a32 = float(np.sum(r32["gc"] * dc) + r32["gnu"] * dn)
+        e64.append(abs(a64 - fd) / abs(fd))
+        e32.append(abs(a32 - fd) / abs(fd))
+    for r in (r32, r64):
+        r["solver"].close()
+    return {
+        "workload": f"{n}^3 channel, seeded z-cylinder r=12, inlet {nc}x{nc} control (3x3 node "
+                    f"blocks), nu={nu0:g}, dt={d.dt:g}, {steps} steps, G={G}, checkpoint every {every}, "
+                    "loss = mean KE over a 16^3 probe",
+        "forward_ms": r32["forward_s"] * 1e3,
+        "recorded_forward_ms": r32["recorded_forward_s"] * 1e3,
+        "backward_ms": r32["backward_s"] * 1e3,
+        "adjoint_forward_ratio": r32["backward_s"] / r32["forward_s"],
+        "loss_fp32": r32["loss"],
+        "loss_fp64": r64["loss"],
+        "fd_directions": ndir,
+        "fd_eps": eps,
+        "max_rel_err_fp64_adjoint_vs_fd": max(e64),
+        "max_rel_err_fp32_adjoint_vs_fd": max(e32),
+        "median_rel_err_fp32_adjoint_vs_fd": float(np.median(e32)),
+    }
+
+
 def cpu_baseline(dom, budget_s: float = 20.0):
     """Oracle (C++/OpenMP port of the reference CPU path) on a bounded sample:
     whole FP32 time steps of the same workload until `budget_s` is spent."""
@@ -203,6 +339,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the inlet-control gradient workload and print its JSON")
+    ap.add_argument("--adjoint-dirs", type=int, default=16,
+                    help="finite-difference directions of the gradient check")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="only run the steps (no events, no autotuning, no JSON line)")
@@ -223,6 +364,9 @@ def main():
     lib()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.adjoint_only:
+        print(json.dumps(adjoint_workload(ndir=args.adjoint_dirs)), flush=True)
+        return
     dom = domains.ellipsoids(args.size)
     fused_ok = True
     try:
@@ -247,6 +391,7 @@ def main():
     if head_path == "fused" and not args.no_plain:
         plain = time_path(dom, CMCX_PATH_PLAIN, max(2, args.steps // 4), 1)
     batch = None if args.no_batch else time_batch(max(4, args.steps // 2), 2)
+    adjoint = None if args.no_adjoint else adjoint_workload(ndir=args.adjoint_dirs)
     cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
 
     achieved = head["sweep_achieved_gbs"]
@@ -309,6 +454,8 @@ def main():
             "plain_value": bp["cell_updates_per_s"],
             "speedup_over_plain": bf["cell_updates_per_s"] / bp["cell_updates_per_s"],
         }
+    if adjoint is not None:
+        line["adjoint"] = adjoint
     if plain is not None:
         line["plain"] = {
             "value": plain["cell_updates_per_s"],

@@ -247,7 +247,7 @@ def batch_boxes(count: int = 64, seed: int = 0, edges=(32, 48, 64, 96), dt: floa
 
 def cylinder_channel(n: int = 96, radius: float = 12.0, seed: int = 0, nu: float = 1e-3,
                      control: np.ndarray | None = None, steps: int = 200,
-                     dt: float = 5e-3, num_global: int = 4, u0: float = 1.0) -> tuple[Domain3D, np.ndarray]:
+                     dt: float = 3e-3, num_global: int = 4, u0: float = 1.0) -> tuple[Domain3D, np.ndarray]:
     """96^3 channel with a seeded z-axis cylinder; inlet driven by a 32x32 control.
 
     Cylinder centre (x, y) uniform in the upstream half (x in [r+4, n/2],
@@ -255,11 +255,12 @@ def cylinder_channel(n: int = 96, radius: float = 12.0, seed: int = 0, nu: float
     over 3x3 cell blocks; initial control 1 + 0.05 N(0,1) (default_rng(seed));
     outlet x = n-1 NODE_VALVE BC_FREE; other faces no-slip.  Re = 1/nu.
     Initial state: u = u0 on NODE_IN nodes (an impulsively started cylinder in
-    a uniform stream), v = w = 0, T = baseT.  dt = 5e-3 makes the 200 steps
-    one flow-through time (u t = 1 = channel length), so that the inlet
-    control reaches the downstream probe within the rollout; starting from
-    rest, the flow would not enter beyond a few cells (the momentum rows carry
-    no pressure projection, only viscous and T-gradient coupling).
+    a uniform stream), v = w = 0, T = baseT.  With dt = 3e-3 the 200 steps
+    carry the inlet control 0.6 channel lengths downstream (u t = 0.6), past
+    the cylinder into the probe behind it; starting from rest, the flow would
+    not enter beyond a few cells (the momentum rows carry no pressure
+    projection, only viscous and T-gradient coupling).  dt = 5e-3 overflows
+    after about 150 steps at the stagnation point in front of the cylinder.
     """
     rng = np.random.default_rng(seed)
     d = box(n, n, n)
