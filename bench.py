@@ -1,21 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the ADI time step on one MI355X.
 
-Workload: the 256^3 irregular domain with 40 seeded ellipsoids removed (FP32,
-inlet u=1, outlet BC_FREE, Re=1000, dt=2e-4, 4 Picard iterations, 1 local
-sweep).  A step is UpdateBoundaries + TimeStep over the whole grid, with the
-state resident in HBM.
+Headline workload: the 256^3 irregular domain with 40 seeded ellipsoids
+removed (FP32, inlet u=1, outlet BC_FREE, Re=1000, dt=2e-4, 4 Picard
+iterations, 1 local sweep).  A step is UpdateBoundaries + TimeStep over the
+whole grid, with the state resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|plain]
 
 The timed region is bracketed by device synchronisations of the HIP runtime the
 solver runs on (torch is not imported: it bundles a second HIP runtime).
 
-Prints one JSON line: cell-updates/s of the headline path, the roofline of the
-dominant kernel (the sweep: algorithmic bytes (16w+1)*S_d per sweep over the
-measured sweep-kernel time from HIP events on the solver's stream), the plain
-per-axis-launch path for comparison, and the oracle (C++/OpenMP restatement of
-the reference CPU path) timed on a bounded sample on the host cores.
+Prints one JSON line:
+  value / ms_per_step  cell-updates/s of the headline path (fused sweeps; the
+                       faster of direct launches and graph replay)
+  roofline             the dominant kernel, the sweep: algorithmic bytes
+                       (16w+1)*S_d per sweep over the sweep-kernel time from HIP
+                       events on the solver's stream; traffic from the committed
+                       PMC summary
+  cpu_baseline         the oracle (C++/OpenMP restatement of the reference CPU
+                       path) on a bounded sample, on the host cores
+  plain                the per-segment, per-axis-launch GPU path
+  batch                64 independent boxes advanced together (one launch per
+                       sweep) against the plain path on the same grids
+  adjoint              gradient of a probe's kinetic energy after 200 steps of
+                       a 96^3 channel w.r.t. a 32x32 inlet control and the
+                       viscosity: backward/forward time and finite-difference check
 """
 from __future__ import annotations
 
@@ -75,12 +85,15 @@ def copy_calibration(mib: int):
     h.hipFree(b)
 
 
-def run_plain_steps(dom, path: int, steps: int, warmup: int, dtype: str = "f32"):
+def run_plain_steps(dom, path: int, steps: int, warmup: int, dtype: str = "f32", variants=None):
     """Steps only, no HIP events and no autotuning (for PMC counter runs, where the
-    profiler does not support event timing)."""
+    profiler does not support event timing); `variants` fixes the tile variants."""
     from cmcx.solver import make_solver
 
     s = make_solver(dom, dtype, path=path)
+    if variants and path == 1:
+        for a, v in enumerate(variants):
+            s.set_tile_variant(a, v)
     for i in range(warmup + steps):
         s.UpdateBoundaries()
         s.TimeStep(dom.dt, dom.num_global, dom.num_local, False)
@@ -301,6 +314,18 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
     }
 
 
+def measured_traffic(size: int, path: str):
+    """HBM bytes per sweep launch from the committed PMC summary of this
+    workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy), or None."""
+    fn = os.path.join(ROOT, "profiles", "pmc_sweeps_ellipsoids256_fp32.md")
+    if size != 256 or path != "fused" or not os.path.exists(fn):
+        return None
+    for line in open(fn):
+        if line.startswith("traffic_bytes_per_sweep_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(dom, budget_s: float = 20.0):
     """Oracle (C++/OpenMP port of the reference CPU path) on a bounded sample:
     whole FP32 time steps of the same workload until `budget_s` is spent."""
@@ -347,6 +372,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="only run the steps (no events, no autotuning, no JSON line)")
+    ap.add_argument("--tile-variants", default="0,1,0",
+                    help="pmc-mode: fused kernel variants of the X, Y, Z sweeps (the autotuner's "
+                         "choice on the 256^3 workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference kernel for "
                          "PMC byte counters)")
@@ -379,7 +407,8 @@ def main():
     head_path = args.path or ("fused" if fused_ok else "plain")
     if args.pmc_mode:
         run_plain_steps(dom, CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN,
-                        args.steps, args.warmup)
+                        args.steps, args.warmup,
+                        variants=[int(v) for v in args.tile_variants.split(",")])
         print(f"pmc-mode: {args.steps} steps of the {head_path} path done", flush=True)
         return
     head_id = CMCX_PATH_FUSED if head_path == "fused" else CMCX_PATH_PLAIN
@@ -429,7 +458,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.size, head_path),
+            "traffic_source": "profiles/pmc_sweeps_ellipsoids256_fp32.md",
             "kernel": "sweep",
             "bytes_per_step": head["sweep_bytes_per_step"],
             "sweep_ms_per_launch": head["sweep_ms_per_launch"],
