@@ -36,8 +36,21 @@
 // private buffers and the endpoints are scattered serially in list order, so
 // the later segment of the list wins, independent of the thread count.
 //
-// Parity status per function: see the oracle section of DESIGN.md in the
-// repository root (which checks pin which function).
+// What pins each function (tests/test_oracle_pins.py, test_oracle_adjoint.py):
+//   gen_segments        numpy run-length brute force, known answers, invariants,
+//                       Z-block permutation
+//   solve_tridiagonal   bit-exact against the reference's Algorithms.h compiled
+//                       from the reference tree (oracle/_ref), and a dense solve
+//   FluidParams         bit-exact against the reference's Geometry.h (oracle/_ref)
+//   build_matrix, BCs,  closed forms of single steps (diffusion eigenmode decay,
+//   stencils, diss_*    pressure and advection signs per axis, every dissipation
+//                       term) and a dense brute force of whole steps
+//                       (tests/brute_adi.py, written from the model)
+//   time_step, merge,   dense brute force of whole steps, both merge modes;
+//   update_boundaries   zero state stays zero
+//   div_error           closed form (linear field)
+//   get_layer           closed form (index-valued state, NODE_OUT fill)
+//   Adjoint             central differences of time_step in every input group
 
 #include <cmath>
 #include <cstdint>

@@ -371,3 +371,37 @@ def test_divergence_error_closed_form():
     o = oracle_lib.Oracle3D(d, "f64")
     o.set_fields(u, np.zeros(d.n), np.zeros(d.n), d.T)
     assert np.isclose(o.div_error(), 0.5 * d.dx**3, rtol=1e-12)
+
+
+# ------------------------------------------------------------- output view ----
+def test_get_layer_closed_form():
+    """GetLayer (Solver3D.cpp:21-25 -> FilterToArrays): output node (i, j, k)
+    takes grid node (i*nx/ox, j*ny/oy, k*nz/oz) of the layer that held the
+    state before the last step; NODE_OUT nodes read 99999.  An index-valued
+    state makes every pick visible."""
+    d = domains.box(9, 7, 8)
+    t = d.type.reshape(d.shape)
+    t[4, 3, 0] = domains.NODE_OUT  # a face node, not next to NODE_IN
+    o = oracle_lib.Oracle3D(d, "f64")
+    o.create_segments()
+    ids = np.arange(d.n, dtype=np.float64)
+    o.set_fields(ids, 2 * ids, 3 * ids, 4 * ids)
+    o.update_boundaries()
+    o.step(1e-3, 1, 1, False)  # the index state moves to the `next` buffer
+    # that layer is the index state with the node values written on BOUND/VALVE
+    # nodes by UpdateBoundaries
+    bv = (d.type == domains.NODE_BOUND) | (d.type == domains.NODE_VALVE)
+    U = np.where(bv, d.vx, ids)
+    V = np.where(bv, d.vy, 2 * ids)
+    T4 = np.where(bv, d.T, 4 * ids)
+    for ox, oy, oz in ((9, 7, 8), (4, 3, 5), (12, 10, 9)):
+        vel, T = o.get_layer(ox, oy, oz)
+        i = np.arange(ox)[:, None, None] * d.nx // ox
+        j = np.arange(oy)[None, :, None] * d.ny // oy
+        k = np.arange(oz)[None, None, :] * d.nz // oz
+        src = ((i * d.ny + j) * d.nz + k).ravel()
+        out = d.type[src] == domains.NODE_OUT
+        assert out.any() or (ox, oy, oz) != (9, 7, 8)
+        assert np.array_equal(vel[:, 0], np.where(out, 99999.0, U[src]).astype(np.float32))
+        assert np.array_equal(vel[:, 1], np.where(out, 99999.0, V[src]).astype(np.float32))
+        assert np.array_equal(T, np.where(out, 99999.0, T4[src]))
