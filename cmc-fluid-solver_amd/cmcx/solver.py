@@ -254,6 +254,66 @@ class AdiSolver3D:
             pass
 
 
+class AdiSolver2D:
+    """The reference's 2D ADI solver (src/FluidSolver2D/AdiSolver2D.h, Solver2D.h)
+    over the C ABI: Init, UpdateBoundaries, TimeStep (returns (err, iters); raises
+    cmcx._lib.CmcxError with CMCX_ERR_PICARD where the reference exits), plus
+    fields() = (u, v, p, T)."""
+
+    def __init__(self, dtype: str = "f64"):
+        self.h = ctypes.c_void_p(None)
+        self.dtype = _lib.CMCX_F32 if dtype in ("f32", "float32", np.float32) else _lib.CMCX_F64
+        self.np_dtype = _NP[self.dtype]
+
+    def Init(self, grid, params: FluidParams, stream=None):
+        arrs = [_u8(grid.type), _u8(grid.bc)]
+        vals = [_f64(grid.vx), _f64(grid.vy), _f64(grid.T)]
+        self._keep = arrs + vals
+        dom = _lib.cmcx2d_domain()
+        dom.dims[:] = [grid.nx, grid.ny]
+        dom.h[:] = [grid.dx, grid.dy]
+        dom.start_t = grid.base_t
+        dom.type, dom.bc = (a.ctypes.data_as(_lib._u8p) for a in arrs)
+        dom.v[0] = vals[0].ctypes.data_as(_lib._dp)
+        dom.v[1] = vals[1].ctypes.data_as(_lib._dp)
+        dom.T = vals[2].ctypes.data_as(_lib._dp)
+        prm = _lib.cmcx_params(params.v_T, params.v_vis, params.t_vis, params.t_phi)
+        h = ctypes.c_void_p(None)
+        check(lib().cmcx2d_create(ctypes.byref(dom), ctypes.byref(prm), self.dtype,
+                                  ctypes.c_void_p(stream), ctypes.byref(h)), "cmcx2d_create")
+        self.h = h
+        self.grid = grid
+        self.params = params
+        return self
+
+    def UpdateBoundaries(self):
+        check(lib().cmcx2d_update_boundaries(self.h), "cmcx2d_update_boundaries")
+
+    def TimeStep(self, dt: float, num_global: int, num_local: int = 1):
+        err = ctypes.c_double(0.0)
+        it = ctypes.c_int(0)
+        check(lib().cmcx2d_step(self.h, float(dt), int(num_global), int(num_local), ctypes.byref(err),
+                                ctypes.byref(it)), "cmcx2d_step")
+        return err.value, it.value
+
+    def fields(self):
+        n = self.grid.nx * self.grid.ny
+        out = [np.zeros(n, self.np_dtype) for _ in range(4)]
+        check(lib().cmcx2d_get_fields(self.h, *[a.ctypes.data for a in out]), "cmcx2d_get_fields")
+        return tuple(out)
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().cmcx2d_destroy(self.h)
+            self.h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _domain_struct(grid, keep: list):
     arrs = [_u8(grid.type), _u8(grid.bc_vel), _u8(grid.bc_temp)]
     vals = [_f64(grid.vx), _f64(grid.vy), _f64(grid.vz), _f64(grid.T)]

@@ -131,6 +131,8 @@ class Solver final : public ISolver {
         int nck = 0;
         T* tape = nullptr;     // per step: S, Picard layers 1..3G, sweep outputs 0..3G-1
         int64_t tape_layers = 0;
+        int tape_G = 0;
+        bool last_taped = false;  // the tape holds the last interval of the recording
         T* work = nullptr;     // xbar, tbar, state-adjoint sum, lam, node-value gradient, final state
         double* pp = nullptr;  // per-block parameter partials
         int32_t* tiles[3] = {nullptr, nullptr, nullptr};
@@ -890,17 +892,55 @@ class Solver final : public ISolver {
             adj.nck = nck;
         }
         const T dtt = (T)dt;
-        for (int i = 0; i < steps; i++) {
-            if (i % every == 0) copy_layer(lay(adj.ck, i / every), cur);
-            launch_step(true, dtt, G, 1);
-            std::swap(cur, next);
-        }
-        adj.recorded = true;
         adj.dt = dtt;
         adj.G = G;
         adj.steps = steps;
         adj.every = every;
+        ensure_tape(true);
+        // the last interval is recorded on the way (its layers go to the tape
+        // instead of the step buffers), so the backward pass recomputes only
+        // the intervals before it
+        const int last0 = (nck - 1) * every;
+        const int64_t per = 1 + 6 * (int64_t)G;
+        for (int i = 0; i < steps; i++) {
+            if (i % every == 0) copy_layer(lay(adj.ck, i / every), cur);
+            if (i >= last0) {
+                record_step(adj.tape + (i - last0) * per * 4 * NT);
+            } else {
+                launch_step(true, dtt, G, 1);
+                std::swap(cur, next);
+            }
+        }
+        adj.recorded = true;
+        adj.last_taped = true;
         return CMCX_OK;
+    }
+
+    // Tape of one checkpoint interval; the Picard-layer slots hold the node
+    // values on non-IN nodes (the forward's temp holds the boundary-filled state
+    // there, and the sweeps write NODE_IN nodes only).
+    void ensure_tape(bool init_slots) {
+        const int64_t per = 1 + 6 * (int64_t)adj.G;
+        const int64_t nsteps_tape = std::min(adj.every, adj.steps);
+        const int64_t need = per * nsteps_tape;
+        if (need > adj.tape_layers || adj.tape_G != adj.G) {
+            hipFree(adj.tape);
+            adj.tape = nullptr;
+            CMCX_HIP(hipMalloc((void**)&adj.tape, sizeof(T) * 4 * NT * need));
+            adj.tape_layers = need;
+            adj.tape_G = adj.G;
+            init_slots = true;
+        }
+        if (init_slots) {  // node values may have changed since the last recording
+            Layer4<T> gl;
+            for (int f = 0; f < 4; f++) gl.f[f] = d_g[f];
+            for (int64_t i = 0; i < adj.tape_layers / per; i++)
+                for (int k = 1; k <= 3 * adj.G; k += 2) {
+                    Layer4<T> a = lay(adj.tape + i * per * 4 * NT, k);
+                    Layer4<T> b = k + 1 <= 3 * adj.G ? lay(adj.tape + i * per * 4 * NT, k + 1) : a;
+                    launch_prep_step_list<T>(n_nonin, d_nonin, d_type, gl, a, a, b, st);
+                }
+        }
     }
 
     // UpdateBoundaries + one step from `cur`, keeping every layer the reverse
@@ -948,16 +988,9 @@ class Solver final : public ISolver {
             set_error("adjoint_backward before adjoint_forward");
             return CMCX_ERR_STATE;
         }
-        const int G = adj.G;
         const Geom& g = grids[0].g;
-        const int64_t per = 1 + 6 * (int64_t)G;
-        const int64_t need = per * std::min(adj.every, adj.steps);
-        if (need > adj.tape_layers) {
-            hipFree(adj.tape);
-            adj.tape = nullptr;
-            CMCX_HIP(hipMalloc((void**)&adj.tape, sizeof(T) * 4 * NT * need));
-            adj.tape_layers = need;
-        }
+        const int64_t per = 1 + 6 * (int64_t)adj.G;
+        ensure_tape(false);
         if (!adj.work) {
             CMCX_HIP(hipMalloc((void**)&adj.work, sizeof(T) * 4 * NT * 6));
             CMCX_HIP(hipMalloc((void**)&adj.pp, sizeof(double) * 4 * adj_point_blocks()));
@@ -973,23 +1006,14 @@ class Solver final : public ISolver {
             for (int f = 0; f < 4; f++)
                 if (seed[f])
                     CMCX_HIP(hipMemcpyAsync(XB.f[f], seed[f], sizeof(T) * g.N, hipMemcpyHostToDevice, st));
-        // Picard-layer slots: node values on non-IN nodes, as the forward's
-        // temp holds the boundary-filled state there (sweeps write NODE_IN only)
-        Layer4<T> gl;
-        for (int f = 0; f < 4; f++) gl.f[f] = d_g[f];
-        const int64_t nsteps_tape = std::min(adj.every, adj.steps);
-        for (int64_t i = 0; i < nsteps_tape; i++)
-            for (int k = 1; k <= 3 * G; k += 2) {
-                Layer4<T> a = lay(adj.tape + i * per * 4 * NT, k);
-                Layer4<T> b = k + 1 <= 3 * G ? lay(adj.tape + i * per * 4 * NT, k + 1) : a;
-                launch_prep_step_list<T>(n_nonin, d_nonin, d_type, gl, a, a, b, st);
-            }
         const int nseg = (adj.steps + adj.every - 1) / adj.every;
         for (int sgm = nseg - 1; sgm >= 0; sgm--) {
             const int s0 = sgm * adj.every;
             const int k = std::min(adj.every, adj.steps - s0);
-            copy_layer(cur, lay(adj.ck, sgm));
-            for (int i = 0; i < k; i++) record_step(adj.tape + i * per * 4 * NT);
+            if (sgm != nseg - 1 || !adj.last_taped) {
+                copy_layer(cur, lay(adj.ck, sgm));
+                for (int i = 0; i < k; i++) record_step(adj.tape + i * per * 4 * NT);
+            }
             for (int i = k - 1; i >= 0; i--) reverse_step(adj.tape + i * per * 4 * NT);
         }
         if (gbar)
@@ -1003,6 +1027,7 @@ class Solver final : public ISolver {
         std::vector<double> pp(4 * adj_point_blocks());
         CMCX_HIP(hipMemcpyAsync(pp.data(), adj.pp, sizeof(double) * pp.size(), hipMemcpyDeviceToHost, st));
         copy_layer(cur, FIN);
+        adj.last_taped = nseg == 1;  // the tape now holds the first interval
         CMCX_HIP(hipStreamSynchronize(st));
         if (pbar) {
             for (int q = 0; q < 4; q++) pbar[q] = 0;
@@ -1021,6 +1046,7 @@ class Solver final : public ISolver {
         adj.nck = 0;
         adj.tape_layers = 0;
         adj.recorded = false;
+        adj.last_taped = false;
         return CMCX_OK;
     }
 

@@ -37,7 +37,7 @@ namespace {
 using namespace lsolve;
 
 constexpr int kAdjChunk = 8;
-constexpr int kPointBlocks = 1024;  // fixed grid of adj_point (per-block parameter partials)
+constexpr int kPointBlocks = 2048;  // fixed grid of adj_point (per-block parameter partials)
 
 template <class T>
 struct Coef {
@@ -192,6 +192,132 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
     }
 }
 
+// Transposed line solve along z in the contiguous layout of the forward Z
+// sweep: 32 lanes per line, lane l owns rows [l*M, l*M+M) (16-byte vector
+// accesses), two lines per wavefront, kZWaves wavefronts per work-group.  The
+// coefficients of the rows either side of a chunk come from the neighbouring
+// lanes by shuffle (c of the previous row, a of the next row).
+template <class T, int M>
+__global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
+    Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
+    const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xb, Layer4<T> tb, T beta,
+    Layer4<T> lam, Layer4<T> gb, const int32_t* __restrict__ lines, int vec_ok) {
+    extern __shared__ char smem_raw[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int n = g.nz;
+    const int used = (n + M - 1) / M;
+    const RedLayout L{2 * used, kRedFields * 2 * used + 1};
+    T* red = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * 2 * L.pitch;
+    const int sub = lane >> 5, ll = lane & 31;
+    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int lid = lines[b * (2 * nw) + 2 * wave + sub];
+    const bool live = lid >= 0;
+    const int64_t base = live ? (int64_t)lid * g.nz : 0;
+    const int r0 = ll * M;
+    const bool ok = live && ll < used;
+    const int valid = ok ? min(M, n - r0) : 0;
+    const bool vec = vec_ok && valid == M;
+    const int64_t c0 = base + r0;
+
+    uint8_t ty[M + 2];
+#pragma unroll
+    for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? type[c0 + j] : kOutside;
+    {
+        const int prev = __shfl((int)ty[M], lane - 1, 64);
+        const int nxt = __shfl((int)ty[1], lane + 1, 64);
+        ty[0] = (ll > 0 && ok) ? (uint8_t)prev : kOutside;
+        ty[M + 1] = (ll < 31 && ok && r0 + M < n) ? (uint8_t)nxt : kOutside;
+    }
+    int role[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
+    T w[M];
+    load_run<T, M>(tin.f[2] + c0, w, vec, valid);
+    Coef<T> k[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        bool fv = false, ft = false;
+        if (role[j] != RL_IN && role[j] != RL_NONE) {
+            fv = bcv[c0 + j] == BC_FREE;
+            ft = bct[c0 + j] == BC_FREE;
+        }
+        k[j] = row_coef<T>(role[j], fv, ft, role[j] == RL_IN ? w[j] * p.inv_2h : (T)0, p);
+    }
+    // c of row r0-1 and a of row r0+M (zero outside the line: identity rows)
+    T cvm = __shfl(k[M - 1].cv, lane - 1, 64), ctm = __shfl(k[M - 1].ct, lane - 1, 64);
+    T avp = __shfl(k[0].av, lane + 1, 64), atp = __shfl(k[0].at, lane + 1, 64);
+    if (!(ll > 0 && ok)) cvm = ctm = 0;
+    if (!(ll < 31 && ok && r0 + M < n)) avp = atp = 0;
+
+    T rhs[4][M];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        T xa[M], ta[M];
+        load_run<T, M>(xb.f[v] + c0, xa, vec, valid);
+        load_run<T, M>(tb.f[v] + c0, ta, vec, valid);
+#pragma unroll
+        for (int j = 0; j < M; j++) rhs[v][j] = role[j] == RL_IN ? xa[j] + beta * ta[j] : (T)0;
+    }
+    Chunk<T, M> ch;
+    Row<T> rfirst, rlast;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        Row<T> r;
+        r.av = j == 0 ? cvm : k[j - 1].cv;
+        r.at = j == 0 ? ctm : k[j - 1].ct;
+        r.bv = k[j].bv;
+        r.bt = k[j].bt;
+        r.cv = j == M - 1 ? avp : k[j + 1].av;
+        r.ct = j == M - 1 ? atp : k[j + 1].at;
+#pragma unroll
+        for (int v = 0; v < 4; v++) r.d[v] = rhs[v][j];
+        if (j == 0) rfirst = r;
+        else if (j == M - 1) rlast = r;
+        else chunk_down<T, M>(ch, j, r);
+    }
+    if (M >= 3) chunk_up<T, M>(ch);
+    if (ll < used) {
+        T f[kRedFields], l[kRedFields];
+        chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
+#pragma unroll
+        for (int q = 0; q < kRedFields; q++) {
+            red[L.at(sub, q, 2 * ll)] = f[q];
+            red[L.at(sub, q, 2 * ll + 1)] = l[q];
+        }
+    }
+    __syncthreads();
+    solve_reduced_lines<T>(red, L, 2, 0, 1);
+    __syncthreads();
+    if (!ok) return;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const T xf = red[L.at(sub, 6 + v, 2 * ll)];
+        const T xl = red[L.at(sub, 6 + v, 2 * ll + 1)];
+        T x[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            if (j == 0) x[j] = xf;
+            else if (j == M - 1) x[j] = xl;
+            else {
+                const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
+                const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
+                x[j] = ch.E[v][j] + F * xf + G * xl;
+            }
+        }
+        store_run<T, M>(lam.f[v] + c0, x, vec, valid);
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            const int rl = role[j];
+            if (rl == RL_IN || rl == RL_NONE || j >= valid) continue;
+            bool use = true;
+            if (rl == RL_START || rl == RL_END) use = (v < 3 ? bcv[c0 + j] : bct[c0 + j]) != BC_FREE;
+            if (use) gb.f[v][c0 + j] += x[j];
+        }
+    }
+}
+
 // Pointwise part of a reversed sweep (see the file comment).  Grid-stride over
 // all nodes with a fixed grid, so that block b always owns partial-sum slot b.
 template <class T, int AXIS>
@@ -268,69 +394,65 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         }
         // transposed stencils of the rows that read node q: row c = q - e
         // contributes with +, row c = q + e with -
-        const int pos[3] = {x, y, z};
-        const int dim[3] = {g.nx, g.ny, g.nz};
-        const int64_t str[3] = {sx, sy, 1};
-#pragma unroll
-        for (int e = 0; e < 3; e++)
-#pragma unroll
-            for (int sg = -1; sg <= 1; sg += 2) {
-                const int pc = pos[e] + sg;
-                if (pc < 0 || pc >= dim[e]) continue;
-                const int64_t c = q + sg * str[e];
-                if (type[c] != NODE_IN) continue;
-                const T fsg = sg < 0 ? (T)1 : (T)-1;
-                const T k = p.t_phi * lam.f[3][c];
-                if (AXIS == AX_X) {
-                    if (e == 0) {
-                        const T u_x = (U[c + sx] - U[c - sx]) * ix, v_x = (V[c + sx] - V[c - sx]) * ix;
-                        const T w_x = (W[c + sx] - W[c - sx]) * ix, u_y = (U[c + sy] - U[c - sy]) * iy;
-                        const T u_z = (U[c + 1] - U[c - 1]) * iz;
-                        acc[0] += fsg * (k * 4 * u_x * ix);
-                        acc[1] += fsg * (k * (2 * v_x + u_y) * ix);
-                        acc[2] += fsg * (k * (2 * w_x + u_z) * ix);
-                        acc[3] += fsg * (-p.v_T * lam.f[0][c] * p.inv_2h);
-                    } else if (e == 1) {
-                        const T v_x = (V[c + sx] - V[c - sx]) * ix;
-                        acc[0] += fsg * (k * v_x * iy);
-                    } else {
-                        const T w_x = (W[c + sx] - W[c - sx]) * ix;
-                        acc[0] += fsg * (k * w_x * iz);
-                    }
-                } else if (AXIS == AX_Y) {
-                    if (e == 1) {
-                        const T u_y = (U[c + sy] - U[c - sy]) * iy, v_y = (V[c + sy] - V[c - sy]) * iy;
-                        const T w_y = (W[c + sy] - W[c - sy]) * iy, v_x = (V[c + sx] - V[c - sx]) * ix;
-                        const T v_z = (V[c + 1] - V[c - 1]) * iz;
-                        acc[0] += fsg * (k * (2 * u_y + v_x) * iy);
-                        acc[1] += fsg * (k * 4 * v_y * iy);
-                        acc[2] += fsg * (k * (2 * w_y + v_z) * iy);
-                        acc[3] += fsg * (-p.v_T * lam.f[1][c] * p.inv_2h);
-                    } else if (e == 0) {
-                        const T u_y = (U[c + sy] - U[c - sy]) * iy;
-                        acc[1] += fsg * (k * u_y * ix);
-                    } else {
-                        const T w_y = (W[c + sy] - W[c - sy]) * iy;
-                        acc[1] += fsg * (k * w_y * iz);
-                    }
+        // one neighbour per iteration (not unrolled: keeps the loads of one
+        // neighbour in flight together without holding all six in registers)
+#pragma unroll 1
+        for (int nb = 0; nb < 6; nb++) {
+            const int e = nb >> 1;
+            const int sg = (nb & 1) ? 1 : -1;
+            const int pe = e == 0 ? x : e == 1 ? y : z;
+            const int de = e == 0 ? g.nx : e == 1 ? g.ny : g.nz;
+            const int64_t se = e == 0 ? sx : e == 1 ? sy : 1;
+            const int pc = pe + sg;
+            if (pc < 0 || pc >= de) continue;
+            const int64_t c = q + sg * se;
+            if (type[c] != NODE_IN) continue;
+            const T fsg = sg < 0 ? (T)1 : (T)-1;
+            const T k = p.t_phi * lam.f[3][c];
+            if (AXIS == AX_X) {
+                if (e == 0) {
+                    const T u_x = (U[c + sx] - U[c - sx]) * ix, v_x = (V[c + sx] - V[c - sx]) * ix;
+                    const T w_x = (W[c + sx] - W[c - sx]) * ix, u_y = (U[c + sy] - U[c - sy]) * iy;
+                    const T u_z = (U[c + 1] - U[c - 1]) * iz;
+                    acc[0] += fsg * (k * 4 * u_x * ix);
+                    acc[1] += fsg * (k * (2 * v_x + u_y) * ix);
+                    acc[2] += fsg * (k * (2 * w_x + u_z) * ix);
+                    acc[3] += fsg * (-p.v_T * lam.f[0][c] * p.inv_2h);
+                } else if (e == 1) {
+                    acc[0] += fsg * (k * ((V[c + sx] - V[c - sx]) * ix) * iy);
                 } else {
-                    if (e == 2) {
-                        const T u_z = (U[c + 1] - U[c - 1]) * iz, v_z = (V[c + 1] - V[c - 1]) * iz;
-                        const T w_z = (W[c + 1] - W[c - 1]) * iz, w_x = (W[c + sx] - W[c - sx]) * ix;
-                        const T w_y = (W[c + sy] - W[c - sy]) * iy;
-                        acc[0] += fsg * (k * (2 * u_z + w_x) * iz);
-                        acc[1] += fsg * (k * (2 * v_z + w_y) * iz);
-                        acc[2] += fsg * (k * 4 * w_z * iz);
-                        acc[3] += fsg * (-p.v_T * lam.f[2][c] * p.inv_2h);
-                    } else if (e == 0) {
-                        const T u_z = (U[c + 1] - U[c - 1]) * iz;
-                        acc[2] += fsg * (k * u_z * ix);
-                    } else {
-                        const T v_z = (V[c + 1] - V[c - 1]) * iz;
-                        acc[2] += fsg * (k * v_z * iy);
-                    }
+                    acc[0] += fsg * (k * ((W[c + sx] - W[c - sx]) * ix) * iz);
+                }
+            } else if (AXIS == AX_Y) {
+                if (e == 1) {
+                    const T u_y = (U[c + sy] - U[c - sy]) * iy, v_y = (V[c + sy] - V[c - sy]) * iy;
+                    const T w_y = (W[c + sy] - W[c - sy]) * iy, v_x = (V[c + sx] - V[c - sx]) * ix;
+                    const T v_z = (V[c + 1] - V[c - 1]) * iz;
+                    acc[0] += fsg * (k * (2 * u_y + v_x) * iy);
+                    acc[1] += fsg * (k * 4 * v_y * iy);
+                    acc[2] += fsg * (k * (2 * w_y + v_z) * iy);
+                    acc[3] += fsg * (-p.v_T * lam.f[1][c] * p.inv_2h);
+                } else if (e == 0) {
+                    acc[1] += fsg * (k * ((U[c + sy] - U[c - sy]) * iy) * ix);
+                } else {
+                    acc[1] += fsg * (k * ((W[c + sy] - W[c - sy]) * iy) * iz);
+                }
+            } else {
+                if (e == 2) {
+                    const T u_z = (U[c + 1] - U[c - 1]) * iz, v_z = (V[c + 1] - V[c - 1]) * iz;
+                    const T w_z = (W[c + 1] - W[c - 1]) * iz, w_x = (W[c + sx] - W[c - sx]) * ix;
+                    const T w_y = (W[c + sy] - W[c - sy]) * iy;
+                    acc[0] += fsg * (k * (2 * u_z + w_x) * iz);
+                    acc[1] += fsg * (k * (2 * v_z + w_y) * iz);
+                    acc[2] += fsg * (k * 4 * w_z * iz);
+                    acc[3] += fsg * (-p.v_T * lam.f[2][c] * p.inv_2h);
+                } else if (e == 0) {
+                    acc[2] += fsg * (k * ((U[c + 1] - U[c - 1]) * iz) * ix);
+                } else {
+                    acc[2] += fsg * (k * ((V[c + 1] - V[c - 1]) * iz) * iy);
                 }
             }
+        }
 #pragma unroll
         for (int f = 0; f < 4; f++) tb.f[f][q] = acc[f];
     }
@@ -373,6 +495,12 @@ __global__ void adj_step_end_kernel(int64_t N, const uint8_t* __restrict__ type,
     }
 }
 
+int adj_z_chunk(int n) {
+    for (int m : {2, 4, 8})
+        if (32 * m >= n) return m;
+    return 0;
+}
+
 int adj_waves(int n, int lpg) {
     const int gpw = 64 / lpg;
     return (n + kAdjChunk * gpw - 1) / (kAdjChunk * gpw);
@@ -388,14 +516,34 @@ int adj_lpg() {
 template <class T>
 bool adjoint_supported(const Geom& g) {
     const int lpg = adj_lpg<T>();
-    for (int n : {g.nx, g.ny, g.nz})
+    for (int n : {g.nx, g.ny})
         if (adj_waves(n, lpg) * 64 > 32 * lpg) return false;
-    return true;
+    return adj_z_chunk(g.nz) != 0;
 }
 
 void build_adj_tiles(const uint8_t* t, const Geom& g, int axis, int lpg, int32_t** d_tiles,
                      int64_t* ntiles, int* ltiles) {
     std::vector<int32_t> items;
+    if (axis == AX_Z) {
+        // z lines x*ny + y holding a NODE_IN node, padded with -1 to whole work-groups
+        for (int x = 0; x < g.nx; x++)
+            for (int y = 0; y < g.ny; y++) {
+                const uint8_t* row = &t[x * g.sx + y * g.sy];
+                bool any = false;
+                for (int z = 0; z < g.nz && !any; z++) any = row[z] == NODE_IN;
+                if (any) items.push_back(x * g.ny + y);
+            }
+        while (items.size() % (2 * kZWaves)) items.push_back(-1);
+        *ntiles = (int64_t)items.size() / (2 * kZWaves);  // work-groups
+        *ltiles = 0;
+        *d_tiles = nullptr;
+        if (!items.empty()) {
+            CMCX_HIP(hipMalloc((void**)d_tiles, sizeof(int32_t) * items.size()));
+            CMCX_HIP(hipMemcpy(*d_tiles, items.data(), sizeof(int32_t) * items.size(),
+                               hipMemcpyHostToDevice));
+        }
+        return;
+    }
     // lines of a tile: z lines (X, Y axes) or y lines (Z axis)
     const int nl = axis == AX_Z ? g.ny : g.nz;
     const int lt = (nl + lpg - 1) / lpg;
@@ -433,18 +581,32 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
                       T alpha, T beta, int zmode, const int32_t* tiles, int64_t ntiles, int ltiles,
                       double* pp, hipStream_t st) {
     constexpr int LPG = 128 / (int)sizeof(T);
-    const int n = axis == AX_X ? g.nx : axis == AX_Y ? g.ny : g.nz;
-    const int waves = adj_waves(n, LPG);
-    const int P = waves * (64 / LPG);
-    const size_t lds = (size_t)LPG * (kRedFields * 2 * P + 1) * sizeof(T);
-    if (ntiles > 0) {
+    if (axis == AX_Z) {
+        if (ntiles > 0) {
+            const int m = adj_z_chunk(g.nz);
+            const int used = (g.nz + m - 1) / m;
+            const size_t lds = (size_t)kZWaves * 2 * (kRedFields * 2 * used + 1) * sizeof(T);
+            const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
+#define LZ(MM)                                                                                  \
+    hipLaunchKernelGGL((adj_z_kernel<T, MM>), dim3((unsigned)ntiles), dim3(64 * kZWaves), lds, st, \
+                       g, p, type, bcv, bct, tin, xb, tb, beta, lam, gb, tiles, vec_ok)
+            if (m == 2) LZ(2);
+            else if (m == 4) LZ(4);
+            else LZ(8);
+#undef LZ
+            CMCX_LAUNCH_CHECK();
+        }
+    } else if (ntiles > 0) {
+        const int n = axis == AX_X ? g.nx : g.ny;
+        const int waves = adj_waves(n, LPG);
+        const int P = waves * (64 / LPG);
+        const size_t lds = (size_t)LPG * (kRedFields * 2 * P + 1) * sizeof(T);
 #define LA(AX)                                                                                   \
     hipLaunchKernelGGL((adj_solve_kernel<T, AX, kAdjChunk, LPG, 32 * LPG>), dim3((unsigned)ntiles), \
                        dim3(64 * waves), lds, st, g, p, type, bcv, bct, tin, xb, tb, beta, lam, gb, \
                        tiles, ltiles)
         if (axis == AX_X) LA(AX_X);
-        else if (axis == AX_Y) LA(AX_Y);
-        else LA(AX_Z);
+        else LA(AX_Y);
 #undef LA
         CMCX_LAUNCH_CHECK();
     }

@@ -231,64 +231,6 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     write_phase<T, M>(ch, role, base, s, r0, xf, xl, tin, out, tout, merges);
 }
 
-// 16-byte vector of consecutive elements.
-template <class T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-    using type = float4;
-    static constexpr int n = 4;
-};
-template <>
-struct Vec16<double> {
-    using type = double2;
-    static constexpr int n = 2;
-};
-
-// r[0..M) <- p[0..M): 16-byte loads when `vec` (all M rows inside the line and
-// aligned), else element loads of the first `valid` rows (zero beyond).
-template <class T, int M>
-__device__ __forceinline__ void load_run(const T* __restrict__ p, T* r, bool vec, int valid) {
-    constexpr int VW = Vec16<T>::n;
-    if constexpr (M % VW == 0) {
-        if (vec) {
-            using V = typename Vec16<T>::type;
-#pragma unroll
-            for (int k = 0; k < M / VW; k++) {
-                V v = reinterpret_cast<const V*>(p)[k];
-                const T* e = reinterpret_cast<const T*>(&v);
-#pragma unroll
-                for (int q = 0; q < VW; q++) r[k * VW + q] = e[q];
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < M; j++) r[j] = j < valid ? p[j] : (T)0;
-}
-
-template <class T, int M>
-__device__ __forceinline__ void store_run(T* __restrict__ p, const T* r, bool vec, int valid) {
-    constexpr int VW = Vec16<T>::n;
-    if constexpr (M % VW == 0) {
-        if (vec) {
-            using V = typename Vec16<T>::type;
-#pragma unroll
-            for (int k = 0; k < M / VW; k++) {
-                V v;
-                T* e = reinterpret_cast<T*>(&v);
-#pragma unroll
-                for (int q = 0; q < VW; q++) e[q] = r[k * VW + q];
-                reinterpret_cast<V*>(p)[k] = v;
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < M; j++)
-        if (j < valid) p[j] = r[j];
-}
-
 // Contiguous lines (Z sweep): 32 lanes per line, two lines per wavefront.  A
 // lane's chunk is contiguous in memory, so it is read and written with 16-byte
 // vector accesses (the wave covers its two lines with back-to-back vectors);

@@ -228,5 +228,63 @@ __device__ __forceinline__ void solve_reduced_lines(T* red, RedLayout L, int nl,
     }
 }
 
+// 16-byte vector of consecutive elements.
+template <class T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+
+// r[0..M) <- p[0..M): 16-byte loads when `vec` (all M rows inside the line and
+// aligned), else element loads of the first `valid` rows (zero beyond).
+template <class T, int M>
+__device__ __forceinline__ void load_run(const T* __restrict__ p, T* r, bool vec, int valid) {
+    constexpr int VW = Vec16<T>::n;
+    if constexpr (M % VW == 0) {
+        if (vec) {
+            using V = typename Vec16<T>::type;
+#pragma unroll
+            for (int k = 0; k < M / VW; k++) {
+                V v = reinterpret_cast<const V*>(p)[k];
+                const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+                for (int q = 0; q < VW; q++) r[k * VW + q] = e[q];
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) r[j] = j < valid ? p[j] : (T)0;
+}
+
+template <class T, int M>
+__device__ __forceinline__ void store_run(T* __restrict__ p, const T* r, bool vec, int valid) {
+    constexpr int VW = Vec16<T>::n;
+    if constexpr (M % VW == 0) {
+        if (vec) {
+            using V = typename Vec16<T>::type;
+#pragma unroll
+            for (int k = 0; k < M / VW; k++) {
+                V v;
+                T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+                for (int q = 0; q < VW; q++) e[q] = r[k * VW + q];
+                reinterpret_cast<V*>(p)[k] = v;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++)
+        if (j < valid) p[j] = r[j];
+}
+
 }  // namespace lsolve
 }  // namespace cmcx
