@@ -391,5 +391,5 @@ def make_solver(dom, dtype: str = "f32", path: int = _lib.CMCX_PATH_PLAIN,
     s.CreateSegments()
     if variant is not None and path == _lib.CMCX_PATH_FUSED:
         for axis in range(3):
-            s.set_tile_variant(axis, variant)
+            s.set_tile_variant(axis, min(variant, 1) if axis == Z else variant)
     return s

@@ -784,7 +784,8 @@ class Solver final : public ISolver {
                 if (plain_axis(a)) continue;
                 float best = 1e30f;
                 int bestv = variant[a];
-                for (int v = 0; v < kFusedVariants; v++) {
+                for (int v = 0; v < (a == AX_Z ? kZVariants : kFusedVariants); v++) {
+                    if (a != AX_Z && !variant_fits(a, v)) continue;
                     const int lv = a == AX_Z ? 0 : v;  // the Z work list does not depend on v
                     ensure_work(a, lv);
                     auto launch = [&]() {
@@ -815,8 +816,20 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    // Longest line of `axis` over the grids fits strided variant v.
+    bool variant_fits(int axis, int v) const {
+        for (const Grid& Gr : grids)
+            if (!fused_variant_ok<T>(axis == AX_X ? Gr.g.nx : Gr.g.ny, v)) return false;
+        return true;
+    }
+
     int set_tile_variant(int axis, int v) override {
-        if (axis < 0 || axis > 2 || v < 0 || v >= kFusedVariants) return CMCX_ERR_ARG;
+        if (axis < 0 || axis > 2 || v < 0 || v >= (axis == AX_Z ? kZVariants : kFusedVariants))
+            return CMCX_ERR_ARG;
+        if (axis != AX_Z && !variant_fits(axis, v)) {
+            set_error("tile variant needs lines of at most 128 nodes");
+            return CMCX_ERR_UNSUPPORTED;
+        }
         variant[axis] = v;
         if (built) prepare_path_buffers();
         clear_graphs();

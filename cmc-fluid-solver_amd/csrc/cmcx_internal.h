@@ -84,10 +84,13 @@ void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const Swee
 
 // Fused sweep (kernels_fused.hip).  A launch covers a batch of independent
 // grids stored back to back in the flat arrays; `variant` selects, for the
-// strided (X, Y) sweeps, the tile width (0 = 128-byte rows, 1 = 64-byte rows;
-// strided_lpg gives its z lines) and, for the Z sweep, the register cap (0 =
-// compiler's choice, 1 = capped for more waves per SIMD).
-constexpr int kFusedVariants = 2;
+// strided (X, Y) sweeps, the tile width (0 = 128-byte rows, 1 = 64-byte rows,
+// 2 = 32-byte rows, each with chunks of 8 rows per lane; 3-5 the same with
+// chunks of 4 rows, for lines of at most 128 nodes; strided_lpg gives the z
+// lines of a tile) and, for the Z sweep, the register cap (0 = compiler's
+// choice, 1 = capped for more waves per SIMD).
+constexpr int kFusedVariants = 6;  // strided sweeps; the Z sweep has 2 (0, 1)
+constexpr int kZVariants = 2;
 constexpr int kZWaves = 4;  // wavefronts per Z-sweep work-group (two lines each)
 
 // Descriptor of one grid of a batch.  three_dt is passed per launch.
@@ -113,6 +116,9 @@ template <class T>
 bool fused_supported(const Geom& g);
 template <class T>
 int strided_lpg(int variant);
+// Strided variant v usable for lines of n nodes.
+template <class T>
+bool fused_variant_ok(int n, int v);
 template <class T>
 void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
                         const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,

@@ -390,18 +390,26 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     }
 }
 
-constexpr int kChunk = 8;  // rows per chunk of the strided kernel
+// Rows per chunk of the strided kernel: 8, or 4 in variants 3-5 (twice the
+// lanes per line, for lines of at most 128 nodes: the reduced system keeps at
+// most 64 rows).
+constexpr int kChunk = 8;
+template <class T>
+constexpr int variant_chunk(int variant) { return variant >= 3 ? 4 : 8; }
 
 // Lines per work-group of a strided-sweep variant: 0 = full 128-byte rows
-// (32 * LPG threads, one work-group per CU), 1 = half rows of 64 bytes with
-// registers capped for two work-groups per CU (two work-groups share each
-// cache line).
+// (32 * LPG threads, one work-group per CU), 1 = half rows of 64 bytes (two
+// work-groups per CU share each cache line), 2 = quarter rows of 32 bytes
+// (four work-groups per CU, in different phases of load, elimination and
+// reduced solve).  Registers are capped at 4 (FP32) / 2 (FP64) waves per SIMD.
 template <class T>
-constexpr int variant_lpg(int variant) { return (variant == 0 ? 128 : 64) / (int)sizeof(T); }
+constexpr int variant_lpg(int variant) {
+    return (variant % 3 == 0 ? 128 : variant % 3 == 1 ? 64 : 32) / (int)sizeof(T);
+}
 
-int strided_waves_for(int n, int lpg) {
+int strided_waves_for(int n, int lpg, int m = kChunk) {
     const int gpw = 64 / lpg;
-    return (n + kChunk * gpw - 1) / (kChunk * gpw);
+    return (n + m * gpw - 1) / (m * gpw);
 }
 
 int z_chunk(int n) {
@@ -417,16 +425,26 @@ int strided_lpg(int variant) {
     return variant_lpg<T>(variant);
 }
 
+// A strided variant fits lines of n nodes when its work-group (one lane per
+// chunk of every line) stays within 32 * LPG threads.
+template <class T>
+bool strided_variant_fits(int n, int v) {
+    const int lpg = variant_lpg<T>(v);
+    return strided_waves_for(n, lpg, variant_chunk<T>(v)) * 64 <= 32 * lpg;
+}
+
 template <class T>
 bool fused_supported(const Geom& g) {
     if (z_chunk(g.nz) == 0 || (sizeof(T) == 8 && z_chunk(g.nz) > 8)) return false;
-    // both variants: at most 32 * LPG threads
-    for (int v = 0; v < 2; v++) {
-        const int lpg = variant_lpg<T>(v);
-        if (strided_waves_for(g.nx, lpg) * 64 > 32 * lpg) return false;
-        if (strided_waves_for(g.ny, lpg) * 64 > 32 * lpg) return false;
-    }
+    // the chunk-8 variants must fit (the chunk-4 ones are optional)
+    for (int v = 0; v < 3; v++)
+        if (!strided_variant_fits<T>(g.nx, v) || !strided_variant_fits<T>(g.ny, v)) return false;
     return true;
+}
+
+template <class T>
+bool fused_variant_ok(int n, int v) {
+    return strided_variant_fits<T>(n, v);
 }
 
 template <class T>
@@ -460,23 +478,37 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
         }
 #undef LZ
     } else {
-        constexpr int LW = variant_lpg<T>(0), LN = variant_lpg<T>(1);
-        const int lpg = variant ? LN : LW;
-        const int waves = strided_waves_for(w.maxn, lpg);
+        constexpr int LW = variant_lpg<T>(0), LN = variant_lpg<T>(1), LQ = variant_lpg<T>(2);
+        const int lpg = variant_lpg<T>(variant);
+        const int mch = variant_chunk<T>(variant);
+        if (!strided_variant_fits<T>(w.maxn, variant))
+            throw std::string("fused sweep: tile variant does not fit the line length");
+        const int waves = strided_waves_for(w.maxn, lpg, mch);
         const int P = waves * (64 / lpg);
         const int pitch = kRedFields * 2 * P + 1;
         size_t lds = (size_t)lpg * pitch * sizeof(T);
-#define LS(AX, LPGV)                                                                             \
-    hipLaunchKernelGGL((fused_strided_kernel<T, AX, kChunk, LPGV, 32 * LPGV,                      \
-                                             (LPGV == LN ? LPGV / 4 : LPGV / 8)>),                \
+#define LS(AX, LPGV, MC)                                                                         \
+    hipLaunchKernelGGL((fused_strided_kernel<T, AX, MC, LPGV, 32 * LPGV, 16 / (int)sizeof(T)>), \
                        dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, type, bcv, bct,  \
                        gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, w.d_items)
         if (axis == AX_X) {
-            if (variant) LS(AX_X, LN);
-            else LS(AX_X, LW);
+            switch (variant) {
+                case 0: LS(AX_X, LW, 8); break;
+                case 1: LS(AX_X, LN, 8); break;
+                case 2: LS(AX_X, LQ, 8); break;
+                case 3: LS(AX_X, LW, 4); break;
+                case 4: LS(AX_X, LN, 4); break;
+                default: LS(AX_X, LQ, 4); break;
+            }
         } else {
-            if (variant) LS(AX_Y, LN);
-            else LS(AX_Y, LW);
+            switch (variant) {
+                case 0: LS(AX_Y, LW, 8); break;
+                case 1: LS(AX_Y, LN, 8); break;
+                case 2: LS(AX_Y, LQ, 8); break;
+                case 3: LS(AX_Y, LW, 4); break;
+                case 4: LS(AX_Y, LN, 4); break;
+                default: LS(AX_Y, LQ, 4); break;
+            }
         }
 #undef LS
     }
@@ -571,6 +603,8 @@ int shared_free_axes(const uint8_t* t, const uint8_t* bv, const uint8_t* bt, con
 
 template bool fused_supported<float>(const Geom&);
 template bool fused_supported<double>(const Geom&);
+template bool fused_variant_ok<float>(int, int);
+template bool fused_variant_ok<double>(int, int);
 template int strided_lpg<float>(int);
 template int strided_lpg<double>(int);
 template void launch_fused_sweep<float>(const InstDesc<float>*, float, int, const FusedWork&,

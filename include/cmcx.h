@@ -196,7 +196,9 @@ int cmcx_profile_read(cmcx_solver* s, double* sweep_ms, int64_t* sweep_launches,
 int cmcx_autotune(cmcx_solver* s, int reps, int* choice);
 /* Forces the fused path's kernel variant of one axis.  X (0) and Y (1) sweeps:
    0 = 128-byte z rows per work-group (one work-group per CU), 1 = 64-byte rows
-   with registers capped for two work-groups per CU.  Z sweep (2): 0 = register
+   (two work-groups per CU), 2 = 32-byte rows (four per CU); 3, 4, 5 = the same
+   with half the rows per lane (lines of at most 128 nodes, else
+   CMCX_ERR_UNSUPPORTED).  Z sweep (2): 0 = register
    use left to the compiler, 1 = capped for more waves per SIMD.  No reference
    counterpart (tuning knob). */
 int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant);

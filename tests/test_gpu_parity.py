@@ -207,12 +207,12 @@ def test_mask_rejected():
         make_solver(d, "f32")
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
 def test_fused_line_lengths(n, variant):
     """Fused kernels at every line length of the benchmark workloads (chunk and
-    tile padding), both tile variants, FP64 against the FP64 oracle, FP32
-    against the FP64 oracle."""
+    tile padding), every tile variant (the chunk-4 ones fit all these lengths),
+    FP64 against the FP64 oracle, FP32 against the FP64 oracle."""
     d = domains.lid_sphere(n, radius=n / 6.0, dt=1e-3)
     say(f"[fused length] {n}^3 variant {variant}")
     for dtype, tol in (("f64", 1e-11), ("f32", 1e-5)):
@@ -241,7 +241,7 @@ def test_fused_anisotropic_grid():
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_fused_variant_ellipsoids(variant):
     d = CASES["ellipsoids_40"]
     say(f"[fused variant] ellipsoids_40 variant {variant}")
