@@ -26,6 +26,8 @@ Prints one JSON line:
   adjoint              gradient of a probe's kinetic energy after 200 steps of
                        a 96^3 channel w.r.t. a 32x32 inlet control and the
                        viscosity: backward/forward time and finite-difference check
+  channel2d            the 2D ADI path on its 256 x 128 channel, GPU beside the
+                       oracle on the host
 """
 from __future__ import annotations
 
@@ -314,6 +316,51 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
     }
 
 
+def channel2d_workload():
+    """The 2D channel (256 x 128, cylinder r = 16, parabolic inlet, Re 100,
+    dt 1e-3, 200 steps, G = 4, FP64): the GPU 2D path timed, against the 2D
+    oracle (the reference's path runs on the CPU) timed on the host, with the
+    final-state error between them."""
+    import oracle_lib
+    from cmcx import domains
+    from cmcx.params import FluidParams
+    from cmcx.solver import AdiSolver2D
+
+    d = domains.channel2d()
+    p = FluidParams.normalized(d.re, 0.72, 1.4)
+    s = AdiSolver2D("f64").Init(d, p)
+    _sync()
+    t0 = time.perf_counter()
+    iters = 0
+    for _ in range(d.steps):
+        s.UpdateBoundaries()
+        iters += s.TimeStep(d.dt, d.num_global, 1)[1]
+    _sync()
+    gpu_s = time.perf_counter() - t0
+    o = oracle_lib.Oracle2D(d, "f64", params=p)
+    t0 = time.perf_counter()
+    for _ in range(d.steps):
+        o.update_boundaries()
+        o.step(d.dt, d.num_global, 1)
+    cpu_s = time.perf_counter() - t0
+    u, v, _, T = s.fields()
+    ou, ov, oT = o.layer(0)
+    err = max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+              for a, b in ((u, ou), (v, ov), (T, oT)))
+    s.close()
+    cells = d.nx * d.ny
+    return {
+        "workload": f"2D channel {d.nx}x{d.ny}, cylinder r=16, Re {d.re:g}, dt {d.dt:g}, "
+                    f"{d.steps} steps, G={d.num_global}, fp64",
+        "gpu_ms_per_step": gpu_s / d.steps * 1e3,
+        "gpu_cell_updates_per_s": cells * d.steps / gpu_s,
+        "cpu_ms_per_step": cpu_s / d.steps * 1e3,
+        "cpu_kind": "port (oracle, 1 thread)",
+        "picard_iterations": iters,
+        "max_rel_l2_vs_oracle": err,
+    }
+
+
 def measured_traffic(size: int, path: str):
     """HBM bytes per sweep launch from the committed PMC summary of this
     workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy), or None."""
@@ -365,6 +412,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-2d", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the inlet-control gradient workload and print its JSON")
     ap.add_argument("--adjoint-dirs", type=int, default=16,
@@ -421,6 +469,7 @@ def main():
         plain = time_path(dom, CMCX_PATH_PLAIN, max(2, args.steps // 4), 1)
     batch = None if args.no_batch else time_batch(max(4, args.steps // 2), 2)
     adjoint = None if args.no_adjoint else adjoint_workload(ndir=args.adjoint_dirs)
+    two_d = None if args.no_2d else channel2d_workload()
     cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
 
     achieved = head["sweep_achieved_gbs"]
@@ -486,6 +535,8 @@ def main():
         }
     if adjoint is not None:
         line["adjoint"] = adjoint
+    if two_d is not None:
+        line["channel2d"] = two_d
     if plain is not None:
         line["plain"] = {
             "value": plain["cell_updates_per_s"],

@@ -827,7 +827,7 @@ class Solver final : public ISolver {
         if (axis < 0 || axis > 2 || v < 0 || v >= (axis == AX_Z ? kZVariants : kFusedVariants))
             return CMCX_ERR_ARG;
         if (axis != AX_Z && !variant_fits(axis, v)) {
-            set_error("tile variant needs lines of at most 128 nodes");
+            set_error("tile variant does not fit the line lengths of these grids");
             return CMCX_ERR_UNSUPPORTED;
         }
         variant[axis] = v;

@@ -108,14 +108,15 @@ __device__ __forceinline__ void boundary_row(int role, int64_t c, const uint8_t*
 // Loads the chunk's rows, eliminates them and returns the two reduced rows.
 // Row r of the line lives at base + r * s; rows outside [0, n) or !ok are
 // identity rows.  role[] receives the row roles.
-template <class T, int AXIS, int M>
+template <class T, int AXIS, int M, bool KEEP>
 __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>& p,
                                             const uint8_t* __restrict__ type,
                                             const uint8_t* __restrict__ bcv,
                                             const uint8_t* __restrict__ bct, const T* const* gv,
                                             const Layer4<T>& in, const Layer4<T>& tin,
                                             int64_t base, int64_t s, int r0, int n, bool ok,
-                                            int* role, Chunk<T, M>& ch, T* f, T* l) {
+                                            int* role, Chunk<T, M>& ch, T* f, T* l, T* keep,
+                                            int kfs, int krs) {
     uint8_t ty[M + 2];
 #pragma unroll
     for (int j = 0; j < M + 2; j++) {
@@ -129,8 +130,16 @@ __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>&
     for (int j = 0; j < M; j++) {
         const int64_t c = base + (int64_t)(r0 + j) * s;
         Row<T> rw;
-        if (role[j] == RL_IN) interior_row<T, AXIS>(g, p, c, s, in, tin, rw);
-        else boundary_row<T>(role[j], c, bcv, bct, gv, rw);
+        if (role[j] == RL_IN) {
+            interior_row<T, AXIS>(g, p, c, s, in, tin, rw);
+            if (KEEP) {
+                // the Picard layer of this row, for the merge of the write phase
+#pragma unroll
+                for (int v = 0; v < 4; v++) keep[v * kfs + j * krs] = tin.f[v][c];
+            }
+        } else {
+            boundary_row<T>(role[j], c, bcv, bct, gv, rw);
+        }
         if (j == 0) rfirst = rw;
         else if (j == M - 1) rlast = rw;
         else chunk_down<T, M>(ch, j, rw);
@@ -140,11 +149,12 @@ __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>&
 }
 
 // Writes the chunk's rows given x_first / x_last of the four fields.
-template <class T, int M>
+template <class T, int M, bool KEEP>
 __device__ __forceinline__ void write_phase(const Chunk<T, M>& ch, const int* role, int64_t base,
                                             int64_t s, int r0, const T* xf, const T* xl,
                                             const Layer4<T>& tin, const Layer4<T>& out,
-                                            const Layer4<T>& tout, int merges) {
+                                            const Layer4<T>& tout, int merges, const T* keep,
+                                            int kfs, int krs) {
 #pragma unroll
     for (int j = 0; j < M; j++) {
         if (role[j] == RL_NONE) continue;
@@ -161,7 +171,8 @@ __device__ __forceinline__ void write_phase(const Chunk<T, M>& ch, const int* ro
             }
             out.f[v][c] = x;
             if (role[j] == RL_IN) {
-                T m = (tin.f[v][c] + x) * (T)0.5;
+                const T t0 = KEEP ? keep[v * kfs + j * krs] : tin.f[v][c];
+                T m = (t0 + x) * (T)0.5;
                 if (merges > 1) m = (m + x) * (T)0.5;
                 tout.f[v][c] = m;
             }
@@ -169,12 +180,17 @@ __device__ __forceinline__ void write_phase(const Chunk<T, M>& ch, const int* ro
     }
 }
 
+// Row pitch (elements) of the LDS copy of the Picard layer: LPG lines plus
+// padding so that the lane groups of a wavefront (M rows apart) start on
+// different banks.
+__host__ __device__ constexpr int keep_pitch(int lpg, int m) { return lpg + (lpg >= m ? lpg / m : 0); }
+
 // ---------------------------------------------------------------------------
 // Strided lines (X or Y sweep).  A work-group owns LPG consecutive z lines and
 // every row of them (MAXT = 32 * LPG threads for lines of up to 256 rows in
 // chunks of M = 8); MINW is the occupancy the register allocation must allow
 // (waves per SIMD), so that two half-width work-groups can share a CU.
-template <class T, int AXIS, int M, int LPG, int MAXT, int MINW>
+template <class T, int AXIS, int M, int LPG, int MAXT, int MINW, bool KEEP>
 __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
     const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
@@ -209,10 +225,17 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
         insts[inst].off + (AXIS == AX_X ? (int64_t)outer * g.sy : (int64_t)outer * g.sx) + z;
     const int r0 = chunk * M;
 
+    // KEEP: each lane parks the Picard layer of its rows in LDS after the
+    // reduced systems, [field][row][line] with a padded line pitch (the lane
+    // groups of a wave then hit different banks); only the lane itself reads it
+    const int krs = keep_pitch(LPG, M);
+    const int kfs = P * M * krs;
+    T* keep = red + LPG * L.pitch + r0 * krs + gl;
     int role[M];
     Chunk<T, M> ch;
     T f[kRedFields], l[kRedFields];
-    chunk_phase<T, AXIS, M>(g, p, type, bcv, bct, gv, in, tin, base, s, r0, n, zok, role, ch, f, l);
+    chunk_phase<T, AXIS, M, KEEP>(g, p, type, bcv, bct, gv, in, tin, base, s, r0, n, zok, role, ch,
+                                  f, l, keep, kfs, krs);
 #pragma unroll
     for (int k = 0; k < kRedFields; k++) {
         red[L.at(gl, k, 2 * chunk)] = f[k];
@@ -228,7 +251,7 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
         xf[v] = red[L.at(gl, 6 + v, 2 * chunk)];
         xl[v] = red[L.at(gl, 6 + v, 2 * chunk + 1)];
     }
-    write_phase<T, M>(ch, role, base, s, r0, xf, xl, tin, out, tout, merges);
+    write_phase<T, M, KEEP>(ch, role, base, s, r0, xf, xl, tin, out, tout, merges, keep, kfs, krs);
 }
 
 // Contiguous lines (Z sweep): 32 lanes per line, two lines per wavefront.  A
@@ -390,22 +413,25 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     }
 }
 
-// Rows per chunk of the strided kernel: 8, or 4 in variants 3-5 (twice the
-// lanes per line, for lines of at most 128 nodes: the reduced system keeps at
-// most 64 rows).
+// Strided-sweep variants: tile width (bytes of z lines per work-group: 128 =
+// one work-group per CU, 64 = two, 32 = four, in different phases of load,
+// elimination and reduced solve), rows per chunk (8, or 4 for lines of at most
+// 128 nodes: twice the lanes per line; the reduced system keeps at most 64
+// rows), and whether the Picard layer read for the coefficients is parked in
+// LDS for the merge instead of re-read from memory.  Registers are capped at 4
+// (FP32) / 2 (FP64) waves per SIMD.
+struct StridedShape {
+    int bytes, chunk;
+    bool keep;
+};
+constexpr StridedShape kShapes[kFusedVariants] = {
+    {128, 8, false}, {64, 8, false}, {32, 8, false}, {128, 4, false}, {64, 4, false},
+    {32, 4, false},  {64, 8, true},  {32, 8, true},  {64, 4, true},   {32, 4, true}};
 constexpr int kChunk = 8;
 template <class T>
-constexpr int variant_chunk(int variant) { return variant >= 3 ? 4 : 8; }
-
-// Lines per work-group of a strided-sweep variant: 0 = full 128-byte rows
-// (32 * LPG threads, one work-group per CU), 1 = half rows of 64 bytes (two
-// work-groups per CU share each cache line), 2 = quarter rows of 32 bytes
-// (four work-groups per CU, in different phases of load, elimination and
-// reduced solve).  Registers are capped at 4 (FP32) / 2 (FP64) waves per SIMD.
+constexpr int variant_chunk(int variant) { return kShapes[variant].chunk; }
 template <class T>
-constexpr int variant_lpg(int variant) {
-    return (variant % 3 == 0 ? 128 : variant % 3 == 1 ? 64 : 32) / (int)sizeof(T);
-}
+constexpr int variant_lpg(int variant) { return kShapes[variant].bytes / (int)sizeof(T); }
 
 int strided_waves_for(int n, int lpg, int m = kChunk) {
     const int gpw = 64 / lpg;
@@ -425,18 +451,30 @@ int strided_lpg(int variant) {
     return variant_lpg<T>(variant);
 }
 
+constexpr size_t kMaxLds = 160 * 1024;
+
+template <class T>
+size_t strided_lds(int n, int v) {
+    const int lpg = variant_lpg<T>(v), m = variant_chunk<T>(v);
+    const int P = strided_waves_for(n, lpg, m) * (64 / lpg);
+    size_t b = (size_t)lpg * (kRedFields * 2 * P + 1) * sizeof(T);
+    if (kShapes[v].keep) b += (size_t)4 * P * m * keep_pitch(lpg, m) * sizeof(T);
+    return b;
+}
+
 // A strided variant fits lines of n nodes when its work-group (one lane per
-// chunk of every line) stays within 32 * LPG threads.
+// chunk of every line) stays within 32 * LPG threads and its LDS within a CU.
 template <class T>
 bool strided_variant_fits(int n, int v) {
     const int lpg = variant_lpg<T>(v);
-    return strided_waves_for(n, lpg, variant_chunk<T>(v)) * 64 <= 32 * lpg;
+    return strided_waves_for(n, lpg, variant_chunk<T>(v)) * 64 <= 32 * lpg &&
+           strided_lds<T>(n, v) <= kMaxLds;
 }
 
 template <class T>
 bool fused_supported(const Geom& g) {
     if (z_chunk(g.nz) == 0 || (sizeof(T) == 8 && z_chunk(g.nz) > 8)) return false;
-    // the chunk-8 variants must fit (the chunk-4 ones are optional)
+    // the chunk-8, memory-read variants must fit (the others are optional)
     for (int v = 0; v < 3; v++)
         if (!strided_variant_fits<T>(g.nx, v) || !strided_variant_fits<T>(g.ny, v)) return false;
     return true;
@@ -478,38 +516,37 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
         }
 #undef LZ
     } else {
-        constexpr int LW = variant_lpg<T>(0), LN = variant_lpg<T>(1), LQ = variant_lpg<T>(2);
         const int lpg = variant_lpg<T>(variant);
         const int mch = variant_chunk<T>(variant);
         if (!strided_variant_fits<T>(w.maxn, variant))
             throw std::string("fused sweep: tile variant does not fit the line length");
         const int waves = strided_waves_for(w.maxn, lpg, mch);
-        const int P = waves * (64 / lpg);
-        const int pitch = kRedFields * 2 * P + 1;
-        size_t lds = (size_t)lpg * pitch * sizeof(T);
-#define LS(AX, LPGV, MC)                                                                         \
-    hipLaunchKernelGGL((fused_strided_kernel<T, AX, MC, LPGV, 32 * LPGV, 16 / (int)sizeof(T)>), \
+        size_t lds = strided_lds<T>(w.maxn, variant);
+#define LS(AX, V)                                                                                 \
+    hipLaunchKernelGGL((fused_strided_kernel<T, AX, kShapes[V].chunk, kShapes[V].bytes / (int)sizeof(T), \
+                                             32 * (kShapes[V].bytes / (int)sizeof(T)),            \
+                                             16 / (int)sizeof(T), kShapes[V].keep>),              \
                        dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, type, bcv, bct,  \
                        gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, w.d_items)
+#define LSA(AX)                        \
+    switch (variant) {                 \
+        case 0: LS(AX, 0); break;      \
+        case 1: LS(AX, 1); break;      \
+        case 2: LS(AX, 2); break;      \
+        case 3: LS(AX, 3); break;      \
+        case 4: LS(AX, 4); break;      \
+        case 5: LS(AX, 5); break;      \
+        case 6: LS(AX, 6); break;      \
+        case 7: LS(AX, 7); break;      \
+        case 8: LS(AX, 8); break;      \
+        default: LS(AX, 9); break;     \
+    }
         if (axis == AX_X) {
-            switch (variant) {
-                case 0: LS(AX_X, LW, 8); break;
-                case 1: LS(AX_X, LN, 8); break;
-                case 2: LS(AX_X, LQ, 8); break;
-                case 3: LS(AX_X, LW, 4); break;
-                case 4: LS(AX_X, LN, 4); break;
-                default: LS(AX_X, LQ, 4); break;
-            }
+            LSA(AX_X);
         } else {
-            switch (variant) {
-                case 0: LS(AX_Y, LW, 8); break;
-                case 1: LS(AX_Y, LN, 8); break;
-                case 2: LS(AX_Y, LQ, 8); break;
-                case 3: LS(AX_Y, LW, 4); break;
-                case 4: LS(AX_Y, LN, 4); break;
-                default: LS(AX_Y, LQ, 4); break;
-            }
+            LSA(AX_Y);
         }
+#undef LSA
 #undef LS
     }
     CMCX_LAUNCH_CHECK();

@@ -207,7 +207,7 @@ def test_mask_rejected():
         make_solver(d, "f32")
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", list(range(10)))
 @pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
 def test_fused_line_lengths(n, variant):
     """Fused kernels at every line length of the benchmark workloads (chunk and
@@ -241,7 +241,7 @@ def test_fused_anisotropic_grid():
     s.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", list(range(10)))
 def test_fused_variant_ellipsoids(variant):
     d = CASES["ellipsoids_40"]
     say(f"[fused variant] ellipsoids_40 variant {variant}")
