@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libcmcx.so")
+# CMCX_LIB overrides the library file (another build of the same sources).
+LIB_PATH = os.environ.get("CMCX_LIB") or os.path.join(os.path.dirname(HERE), "lib", "libcmcx.so")
 
 CMCX_OK = 0
 CMCX_DIVERGED = 1

@@ -881,10 +881,7 @@ class Solver final : public ISolver {
         return r;
     }
 
-    void copy_layer(Layer4<T> dst, Layer4<T> src) {
-        for (int f = 0; f < 4; f++)
-            CMCX_HIP(hipMemcpyAsync(dst.f[f], src.f[f], sizeof(T) * NT, hipMemcpyDeviceToDevice, st));
-    }
+    void copy_layer(Layer4<T> dst, Layer4<T> src) { launch_copy4<T>(dst, src, NT, st); }
 
     int adjoint_forward(double dt, int steps, int G, int every) override {
         if (!built) return CMCX_ERR_STATE;
@@ -989,10 +986,10 @@ class Solver final : public ISolver {
                 Layer4<T> tin = lay(sb, k == 0 ? 0 : k), out = lay(sb, 1 + 3 * G + k);
                 const T alpha = q == 2 ? (T)0.25 : (T)0.5, beta = q == 2 ? (T)0.75 : (T)0.5;
                 launch_adj_sweep<T>(g, a, sweep_params(0, a, adj.dt), d_type, d_bcv, d_bct, tin, out,
-                                    XB, TB, CB, LB, GB, alpha, beta, q == 0 ? 1 : 0, adj.tiles[a],
-                                    adj.ntiles[a], adj.ltiles[a], adj.pp, st);
+                                    XB, TB, CB, LB, GB, alpha, beta, q == 0 ? 1 : 0,
+                                    (q == 0 && it == 0) ? 1 : 0, adj.tiles[a], adj.ntiles[a],
+                                    adj.ltiles[a], adj.pp, st);
             }
-        launch_adj_step_end<T>(NT, d_type, TB, CB, XB, GB, st);
     }
 
     int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,

@@ -148,16 +148,18 @@ int adj_point_blocks();
 // Reverses one sweep: xb/tb (adjoint of the sweep output / of the Picard layer
 // after it) -> lam, tb (before it), xb (adjoint of the sweep input) or, with
 // zmode, cacc += input adjoint and xb = 0; gb += lam on Dirichlet end rows;
-// pp[4*b..] += parameter partials (v_T, v_vis, t_vis, t_phi).
+// pp[4*b..] += parameter partials (v_T, v_vis, t_vis, t_phi).  `last` (the Z
+// sweep of the first Picard iteration) also ends the reversed step: xb = state
+// adjoint, gb += tbar on BOUND/VALVE nodes, tb and cacc cleared.
 template <class T>
 void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
                       Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc, Layer4<T> lam, Layer4<T> gb,
-                      T alpha, T beta, int zmode, const int32_t* tiles, int64_t ntiles, int ltiles,
-                      double* pp, hipStream_t st);
+                      T alpha, T beta, int zmode, int last, const int32_t* tiles, int64_t ntiles,
+                      int ltiles, double* pp, hipStream_t st);
+// dst <- src, four fields of n elements, one launch.
 template <class T>
-void launch_adj_step_end(int64_t N, const uint8_t* type, Layer4<T> tb, Layer4<T> cacc,
-                         Layer4<T> xb, Layer4<T> gb, hipStream_t st);
+void launch_copy4(Layer4<T> dst, Layer4<T> src, int64_t n, hipStream_t st);
 
 // Field kernels (kernels_fields.hip)
 template <class T>

@@ -1,5 +1,5 @@
-// Discrete adjoint of the fused ADI sweep (reverse mode), three kernels per
-// reversed sweep plus one per reversed step.
+// Discrete adjoint of the fused ADI sweep (reverse mode): two kernels per
+// reversed sweep.
 //
 // Forward sweep along axis d, per line (block diagonal over the segments):
 //   M(temp) x = r(in, temp);  out = x on segment nodes;
@@ -18,7 +18,8 @@
 //              inbar = (3/dt) lam (the adjoint of the sweep input, or added to
 //              the state adjoint for a Z sweep, whose input is the state);
 //              per-block partial sums of the FluidParams gradients.
-//   adj_step_end  state adjoint = (Z inputs) + tbar on NODE_IN nodes; tbar on
+//   end of a reversed step (folded into the last adj_point of the step):
+//              state adjoint = (Z inputs) + tbar on NODE_IN nodes; tbar on
 //              BOUND/VALVE nodes goes to the node values (UpdateBoundaries
 //              wrote them into the state and the Picard layer).
 // Row definitions are those of the forward (AdiSolver3D.cpp:755-852,
@@ -325,7 +326,8 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
                                                         const uint8_t* __restrict__ type,
                                                         Layer4<T> tin, Layer4<T> out, Layer4<T> lam,
                                                         Layer4<T> tb, Layer4<T> xb, Layer4<T> cacc,
-                                                        int zmode, T alpha, double* __restrict__ pp) {
+                                                        Layer4<T> gb, int zmode, int last, T alpha,
+                                                        double* __restrict__ pp) {
     const int64_t sx = g.sx, sy = g.sy;
     const int64_t s = AXIS == AX_X ? sx : AXIS == AX_Y ? sy : 1;
     const T ix = p.inv_2hx, iy = p.inv_2hy, iz = p.inv_2hz;
@@ -339,8 +341,9 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         const int x = (int)(q / sx);
         const int y = (int)((q - x * sx) / sy);
         const int z = (int)(q - x * sx - y * sy);
-        const bool inq = type[q] == NODE_IN;
-        T acc[4];
+        const uint8_t tq = type[q];
+        const bool inq = tq == NODE_IN;
+        T acc[4], acc_state[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int f = 0; f < 4; f++) acc[f] = inq ? alpha * tb.f[f][q] : tb.f[f][q];
         if (inq) {
@@ -381,7 +384,10 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const T ib = p.three_dt * l[f];
-                if (zmode) {
+                if (last) {
+                    // the state adjoint is completed below with tbar
+                    acc_state[f] = cacc.f[f][q] + ib;
+                } else if (zmode) {
                     cacc.f[f][q] += ib;
                     xb.f[f][q] = 0;
                 } else {
@@ -453,8 +459,24 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
                 }
             }
         }
+        if (last) {
+            // end of the reversed step (the Picard layer started as the state,
+            // whose BOUND/VALVE nodes UpdateBoundaries filled with node values):
+            // xbar of the previous step, node-value gradients, cleared accumulators
 #pragma unroll
-        for (int f = 0; f < 4; f++) tb.f[f][q] = acc[f];
+            for (int f = 0; f < 4; f++) {
+                if (inq) {
+                    xb.f[f][q] = acc_state[f] + acc[f];
+                    cacc.f[f][q] = 0;
+                } else if (tq == NODE_BOUND || tq == NODE_VALVE) {
+                    gb.f[f][q] += acc[f];
+                }
+                tb.f[f][q] = 0;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; f++) tb.f[f][q] = acc[f];
+        }
     }
     // block sums of the parameter partials, added to this block's slot
     __shared__ double red[4][256 / 64];
@@ -470,28 +492,6 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         double v = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[threadIdx.x][w];
         pp[4 * blockIdx.x + threadIdx.x] += v;
-    }
-}
-
-// End of a reversed step: xbar for the previous step, node-value gradients,
-// and clearing of the per-step accumulators.
-template <class T>
-__global__ void adj_step_end_kernel(int64_t N, const uint8_t* __restrict__ type, Layer4<T> tb,
-                                    Layer4<T> cacc, Layer4<T> xb, Layer4<T> gb) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N) return;
-    const uint8_t t = type[q];
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
-        const T v = tb.f[f][q];
-        if (t == NODE_IN) {
-            xb.f[f][q] = cacc.f[f][q] + v;
-        } else {
-            xb.f[f][q] = 0;
-            if (t == NODE_BOUND || t == NODE_VALVE) gb.f[f][q] += v;
-        }
-        tb.f[f][q] = 0;
-        cacc.f[f][q] = 0;
     }
 }
 
@@ -578,8 +578,8 @@ template <class T>
 void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
                       Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc, Layer4<T> lam, Layer4<T> gb,
-                      T alpha, T beta, int zmode, const int32_t* tiles, int64_t ntiles, int ltiles,
-                      double* pp, hipStream_t st) {
+                      T alpha, T beta, int zmode, int last, const int32_t* tiles, int64_t ntiles,
+                      int ltiles, double* pp, hipStream_t st) {
     constexpr int LPG = 128 / (int)sizeof(T);
     if (axis == AX_Z) {
         if (ntiles > 0) {
@@ -615,19 +615,11 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
     // every launch uses the full fixed grid, so that slot b always belongs to block b
 #define LP(AX)                                                                                  \
     hipLaunchKernelGGL((adj_point_kernel<T, AX>), dim3(kPointBlocks), dim3(256), 0, st, g, p,  \
-                       inv_h2, type, tin, out, lam, tb, xb, cacc, zmode, alpha, pp)
+                       inv_h2, type, tin, out, lam, tb, xb, cacc, gb, zmode, last, alpha, pp)
     if (axis == AX_X) LP(AX_X);
     else if (axis == AX_Y) LP(AX_Y);
     else LP(AX_Z);
 #undef LP
-    CMCX_LAUNCH_CHECK();
-}
-
-template <class T>
-void launch_adj_step_end(int64_t N, const uint8_t* type, Layer4<T> tb, Layer4<T> cacc,
-                         Layer4<T> xb, Layer4<T> gb, hipStream_t st) {
-    hipLaunchKernelGGL((adj_step_end_kernel<T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
-                       N, type, tb, cacc, xb, gb);
     CMCX_LAUNCH_CHECK();
 }
 
@@ -637,9 +629,8 @@ void launch_adj_step_end(int64_t N, const uint8_t* type, Layer4<T> tb, Layer4<T>
     template void launch_adj_sweep<T>(const Geom&, int, const SweepParams<T>&, const uint8_t*,    \
                                       const uint8_t*, const uint8_t*, Layer4<T>, Layer4<T>,       \
                                       Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>, T, T, \
-                                      int, const int32_t*, int64_t, int, double*, hipStream_t);    \
-    template void launch_adj_step_end<T>(int64_t, const uint8_t*, Layer4<T>, Layer4<T>, Layer4<T>, \
-                                         Layer4<T>, hipStream_t);
+                                      int, int, const int32_t*, int64_t, int, double*,             \
+                                      hipStream_t);
 CMCX_ADJ_INST(float)
 CMCX_ADJ_INST(double)
 

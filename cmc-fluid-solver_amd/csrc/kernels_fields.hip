@@ -94,6 +94,14 @@ __global__ void prep_step_list_kernel(int64_t n, const uint32_t* __restrict__ id
     }
 }
 
+template <class T>
+__global__ void copy4_kernel(int64_t n, Layer4<T> dst, Layer4<T> src) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int f = 0; f < 4; f++) dst.f[f][i] = src.f[f][i];
+}
+
 // EvalDivError (TimeLayer3D.h:595-641): per NODE_IN node with i < nx-1, j < ny-1,
 // k < nz-1, |flux divergence| of the dual cell; field-type sums, double result.
 // Grid-stride loop, one (sum, count) partial per block, summed in block order by
@@ -222,6 +230,13 @@ void launch_update_boundaries(const Geom& g, const uint8_t* type, const T* const
 }
 
 template <class T>
+void launch_copy4(Layer4<T> dst, Layer4<T> src, int64_t n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((copy4_kernel<T>), dim3(blocks_for(n)), dim3(kBS), 0, st, n, dst, src);
+    CMCX_LAUNCH_CHECK();
+}
+
+template <class T>
 void launch_update_boundaries_list(int64_t n, const uint32_t* ids, const T* const* gval,
                                    Layer4<T> cur, hipStream_t st) {
     if (n == 0) return;
@@ -303,6 +318,7 @@ void launch_mask_check(const Geom& g, const uint8_t* type, unsigned long long* d
 #define INST(T)                                                                                   \
     template void launch_update_boundaries<T>(const Geom&, const uint8_t*, const T* const*,        \
                                               Layer4<T>, hipStream_t);                             \
+    template void launch_copy4<T>(Layer4<T>, Layer4<T>, int64_t, hipStream_t);                     \
     template void launch_update_boundaries_list<T>(int64_t, const uint32_t*, const T* const*,      \
                                                    Layer4<T>, hipStream_t);                        \
     template void launch_prep_step_list<T>(int64_t, const uint32_t*, const uint8_t*, Layer4<T>,    \

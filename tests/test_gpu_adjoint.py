@@ -67,6 +67,26 @@ def test_adjoint_matches_oracle(small, dtype, path):
     s.close()
 
 
+def test_adjoint_long_lines():
+    """Lines longer than 128 nodes along X and Y (the transposed solve then uses
+    chunks of 8 rows), FP64 against the oracle's adjoint."""
+    rng = np.random.default_rng(31)
+    for shape in ((140, 9, 10), (9, 140, 10)):
+        d = domains.random_domain(shape, rng, 0.08, free_faces=True)
+        d.re, d.dt = 50.0, ac.DT
+        seed = rng.normal(size=(4, d.n)) * (d.type == domains.NODE_IN)
+        say(f"[adjoint long lines] {shape}")
+        ogb, ocb, opb = _oracle_grads(d, seed)
+        s = make_solver(d, "f64", path=FUSED)
+        s.adjoint_forward(STEPS, ac.DT, ac.G, EVERY)
+        gb, sb, pb = s.adjoint_backward(seed)
+        eg, es = rel_l2(gb, ogb), rel_l2(sb, ocb)
+        say(f"  node values {eg:.2e}  initial state {es:.2e}")
+        assert eg < 1e-9 and es < 1e-9
+        assert np.allclose(pb, opb, rtol=1e-9, atol=0)
+        s.close()
+
+
 def _channel(n=36, steps=12):
     # dt 1e-2: inlet changes reach the probe next to the inlet within the run
     d, control = domains.cylinder_channel(n=n, radius=5.0, seed=1, steps=steps, dt=1e-2, num_global=4)
