@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
         }
     }
     __syncthreads();
-    solve_reduced_lines<T>(red, L, LPG, wave, nw);
+    solve_reduced_compact<T>(red, L, LPG, wave, nw);
     __syncthreads();
     if (!ok) return;
     T xf[4], xl[4];
@@ -203,14 +203,11 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
     const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xb, Layer4<T> tb, T beta,
     Layer4<T> lam, Layer4<T> gb, const int32_t* __restrict__ lines, int vec_ok) {
-    extern __shared__ char smem_raw[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const int n = g.nz;
     const int used = (n + M - 1) / M;
-    const RedLayout L{2 * used, kRedFields * 2 * used + 1};
-    T* red = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * 2 * L.pitch;
     const int sub = lane >> 5, ll = lane & 31;
     const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int lid = lines[b * (2 * nw) + 2 * wave + sub];
@@ -279,32 +276,26 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
         else chunk_down<T, M>(ch, j, r);
     }
     if (M >= 3) chunk_up<T, M>(ch);
-    if (ll < used) {
+    // lanes without rows hold identity rows; the line's reduced system stays in
+    // its 32 lanes
+    T xf[4], xl[4];
+    {
         T f[kRedFields], l[kRedFields];
         chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
-#pragma unroll
-        for (int q = 0; q < kRedFields; q++) {
-            red[L.at(sub, q, 2 * ll)] = f[q];
-            red[L.at(sub, q, 2 * ll + 1)] = l[q];
-        }
+        solve_chunks_in_lanes<T>(f, l, ll, used, 32, xf, xl);
     }
-    __syncthreads();
-    solve_reduced_lines<T>(red, L, 2, 0, 1);
-    __syncthreads();
     if (!ok) return;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const T xf = red[L.at(sub, 6 + v, 2 * ll)];
-        const T xl = red[L.at(sub, 6 + v, 2 * ll + 1)];
         T x[M];
 #pragma unroll
         for (int j = 0; j < M; j++) {
-            if (j == 0) x[j] = xf;
-            else if (j == M - 1) x[j] = xl;
+            if (j == 0) x[j] = xf[v];
+            else if (j == M - 1) x[j] = xl[v];
             else {
                 const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
                 const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                x[j] = ch.E[v][j] + F * xf + G * xl;
+                x[j] = ch.E[v][j] + F * xf[v] + G * xl[v];
             }
         }
         store_run<T, M>(lam.f[v] + c0, x, vec, valid);
@@ -584,8 +575,7 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
     if (axis == AX_Z) {
         if (ntiles > 0) {
             const int m = adj_z_chunk(g.nz);
-            const int used = (g.nz + m - 1) / m;
-            const size_t lds = (size_t)kZWaves * 2 * (kRedFields * 2 * used + 1) * sizeof(T);
+            const size_t lds = 0;
             const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
 #define LZ(MM)                                                                                  \
     hipLaunchKernelGGL((adj_z_kernel<T, MM>), dim3((unsigned)ntiles), dim3(64 * kZWaves), lds, st, \

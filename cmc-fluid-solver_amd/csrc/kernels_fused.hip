@@ -17,8 +17,9 @@
 // (every segment node written) and the Picard merge on NODE_IN nodes.
 //
 // Each line is solved by the chunked partition method of line_solve.h (chunks
-// of M rows per lane, reduced system of 2 rows per chunk, at most 64 rows per
-// line, solved by PCR in LDS).  The U, V and W systems share one matrix (same
+// of M rows per lane, reduced system of 2 rows per chunk, compacted by one
+// cyclic-reduction step to one row per chunk, at most 32 rows per line, then
+// PCR across the lanes: through LDS for strided lines, in registers for Z).  The U, V and W systems share one matrix (same
 // bc_vel and viscosity); T has its own.
 //
 // Layouts:
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
         red[L.at(gl, k, 2 * chunk + 1)] = l[k];
     }
     __syncthreads();
-    solve_reduced_lines<T>(red, L, LPG, wave, nw);
+    solve_reduced_compact<T>(red, L, LPG, wave, nw);
     __syncthreads();
     if (!zok) return;
     T xf[4], xl[4];
@@ -259,6 +260,7 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
 // vector accesses (the wave covers its two lines with back-to-back vectors);
 // the along-line neighbours of a chunk come from the adjacent lanes by shuffle
 // and the cross-line derivatives of w from vector loads of the neighbour lines.
+// The reduced system of a line never leaves its 32 lanes (no LDS, no barrier).
 template <class T, int M, int MINW>
 __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
@@ -266,7 +268,6 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
     Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
     const int32_t* __restrict__ lines, const int32_t* __restrict__ winst, int vec_ok) {
-    extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -279,8 +280,6 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     p.three_dt = three_dt;
     const int n = g.nz;
     const int used = (n + M - 1) / M;  // lanes of a line holding rows (<= 32)
-    const RedLayout L{2 * used, kRedFields * 2 * used + 1};
-    T* red = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * 2 * L.pitch;
 
     const int sub = lane >> 5, ll = lane & 31;
     const int lid = lines[b * (2 * nw) + 2 * wave + sub];  // x*ny + y, or -1 (padding)
@@ -367,35 +366,29 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
         else chunk_down<T, M>(ch, j, r);
     }
     if (M >= 3) chunk_up<T, M>(ch);
-    if (ll < used) {
+    // reduced rows (identity for lanes without rows: their rows are all outside)
+    // and the line's reduced solve inside its 32 lanes
+    T xf[4], xl[4];
+    {
         T f[kRedFields], l[kRedFields];
         chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
-#pragma unroll
-        for (int k = 0; k < kRedFields; k++) {
-            red[L.at(sub, k, 2 * ll)] = f[k];
-            red[L.at(sub, k, 2 * ll + 1)] = l[k];
-        }
+        solve_chunks_in_lanes<T>(f, l, ll, used, 32, xf, xl);
     }
-    __syncthreads();
-    solve_reduced_lines<T>(red, L, 2, 0, 1);
-    __syncthreads();
     if (!ok) return;
     bool all_written = true;
 #pragma unroll
     for (int j = 0; j < M; j++) all_written = all_written && role[j] != RL_NONE;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const T xf = red[L.at(sub, 6 + v, 2 * ll)];
-        const T xl = red[L.at(sub, 6 + v, 2 * ll + 1)];
         T x[M], t[M];
 #pragma unroll
         for (int j = 0; j < M; j++) {
-            if (j == 0) x[j] = xf;
-            else if (j == M - 1) x[j] = xl;
+            if (j == 0) x[j] = xf[v];
+            else if (j == M - 1) x[j] = xl[v];
             else {
                 const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
                 const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                x[j] = ch.E[v][j] + F * xf + G * xl;
+                x[j] = ch.E[v][j] + F * xf[v] + G * xl[v];
             }
             // the Picard layer keeps its value off NODE_IN (rewritten unchanged)
             T m = (tv[v][j] + x[j]) * (T)0.5;
@@ -494,9 +487,7 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
     const unsigned blocks = (unsigned)w.nblocks;
     if (axis == AX_Z) {
         const int m = z_chunk(w.maxn);
-        const int used = (w.maxn + m - 1) / m;
-        const int pitch = kRedFields * 2 * used + 1;
-        size_t lds = (size_t)kZWaves * 2 * pitch * sizeof(T);
+        const size_t lds = 0;  // the reduced systems stay in registers
         // variant 1 caps the registers for 4 (FP32) / 2 (FP64) waves per SIMD
         constexpr int MW = sizeof(T) == 8 ? 2 : 4;
 #define LZ(MM, W)                                                                               \

@@ -1,6 +1,6 @@
 // Device building blocks of the line-wide tridiagonal solves (forward sweeps
 // and their adjoint): row roles, the chunked partition elimination and the
-// parallel cyclic reduction of the reduced systems staged in LDS.
+// solve of the reduced systems.
 //
 // Chunked partition method (the Thomas/PCR hybrid of Kim, Wu, Chang and Hwu,
 // "A scalable tridiagonal solver for GPUs", 2011): the rows of a line are cut
@@ -8,10 +8,14 @@
 // elimination inside the chunk expresses every inner row through the chunk's
 // first and last unknowns, x_j = E_j + F_j x_first + G_j x_last; the first and
 // last rows of all chunks form a tridiagonal "reduced" system of 2 rows per
-// chunk, staged through LDS and solved by parallel cyclic reduction with one
-// reduced row per lane and cross-lane shuffles; each lane then reconstructs
-// its inner rows.  Two matrices share the work: matrix 0 (velocity) serves
-// right-hand sides 0..2, matrix 1 (T) right-hand side 3.
+// chunk.  One cyclic-reduction step eliminates the first unknowns (one row per
+// chunk remains), parallel cyclic reduction with one row per lane and
+// cross-lane shuffles solves for the last unknowns, the first ones follow from
+// their rows, and each lane then reconstructs its inner rows.  The reduced rows
+// reach the solving lanes through LDS when a line's chunks span wavefronts,
+// and stay in registers when they are the lanes of one wavefront.  Two
+// matrices share the work: matrix 0 (velocity) serves right-hand sides 0..2,
+// matrix 1 (T) right-hand side 3.
 #pragma once
 
 #include "cmcx_internal.h"
@@ -50,6 +54,12 @@ inline __device__ unsigned xcd_remap(unsigned b, unsigned nb) {
     return xcd < rem ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
 }
 
+// Reciprocal used by the eliminations: FP32 takes the hardware reciprocal
+// (v_rcp_f32, 1 ulp) instead of the ~10-instruction correctly rounded divide;
+// FP64 keeps the exact division.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
 template <class T>
 struct Row {
     T av, bv, cv, at, bt, ct;
@@ -69,15 +79,15 @@ struct Chunk {
 template <class T, int M>
 __device__ __forceinline__ void chunk_down(Chunk<T, M>& ch, int j, const Row<T>& r) {
     if (j == 1) {
-        T iv = 1 / r.bv, it = 1 / r.bt;
+        T iv = rcp(r.bv), it = rcp(r.bt);
         ch.Fv[j] = -r.av * iv; ch.Gv[j] = r.cv * iv;
         ch.Ft[j] = -r.at * it; ch.Gt[j] = r.ct * it;
 #pragma unroll
         for (int v = 0; v < 3; v++) ch.E[v][j] = r.d[v] * iv;
         ch.E[3][j] = r.d[3] * it;
     } else {
-        T iv = 1 / (r.bv - r.av * ch.Gv[j - 1]);
-        T it = 1 / (r.bt - r.at * ch.Gt[j - 1]);
+        T iv = rcp(r.bv - r.av * ch.Gv[j - 1]);
+        T it = rcp(r.bt - r.at * ch.Gt[j - 1]);
 #pragma unroll
         for (int v = 0; v < 3; v++) ch.E[v][j] = (r.d[v] - r.av * ch.E[v][j - 1]) * iv;
         ch.E[3][j] = (r.d[3] - r.at * ch.E[3][j - 1]) * it;
@@ -169,8 +179,8 @@ __device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, 
         }
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            T k1 = A[q] / Bm[q];
-            T k2 = C[q] / Bp[q];
+            T k1 = A[q] * rcp(Bm[q]);
+            T k2 = C[q] * rcp(Bp[q]);
             B[q] = B[q] - Cm[q] * k1 - Ap[q] * k2;
             if (q == 0) {
 #pragma unroll
@@ -182,9 +192,81 @@ __device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, 
             C[q] = -Cp[q] * k2;
         }
     }
+    const T i0 = rcp(B[0]);
 #pragma unroll
-    for (int v = 0; v < 3; v++) D[v] = D[v] / B[0];
-    D[3] = D[3] / B[1];
+    for (int v = 0; v < 3; v++) D[v] = D[v] * i0;
+    D[3] = D[3] * rcp(B[1]);
+}
+
+// Identity reduced row (x = 0) in the kRedFields layout.
+template <class T>
+__device__ __forceinline__ void ident_row(T* r) {
+#pragma unroll
+    for (int k = 0; k < kRedFields; k++) r[k] = (k == 1 || k == 4) ? (T)1 : (T)0;
+}
+
+// One cyclic-reduction step on the reduced system (2 rows per chunk): chunk i's
+// rows f_i (x_{l,i-1}, x_{f,i}, x_{l,i}) and l_i (x_{f,i}, x_{l,i}, x_{f,i+1}),
+// with the next chunk's f_{i+1}, give after eliminating x_{f,i} and x_{f,i+1}
+// row i of a tridiagonal system in the chunks' last unknowns:
+//   A = -m a_f,  B = b_l - m c_f - m' a_f',  C = -m' c_f',  D = d_l - m d_f - m' d_f'
+// with m = a_l / b_f and m' = c_l / b_f' (primes: chunk i+1, an identity row
+// when there is none).  ib receives 1 / b_f per matrix for first_unknowns.
+template <class T>
+__device__ __forceinline__ void compact_rows(const T* f, const T* l, const T* fn, T* A, T* B,
+                                             T* C, T* D, T* ib) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        ib[q] = rcp(f[3 * q + 1]);
+        const T m = l[3 * q] * ib[q];
+        const T mn = l[3 * q + 2] * rcp(fn[3 * q + 1]);
+        A[q] = -m * f[3 * q];
+        B[q] = l[3 * q + 1] - m * f[3 * q + 2] - mn * fn[3 * q];
+        C[q] = -mn * fn[3 * q + 2];
+#pragma unroll
+        for (int v = (q == 0 ? 0 : 3); v < (q == 0 ? 3 : 4); v++)
+            D[v] = l[6 + v] - m * f[6 + v] - mn * fn[6 + v];
+    }
+}
+
+// x_{f,i} from row f_i once the last unknowns of chunks i-1 (xlp) and i (xl)
+// are known.
+template <class T>
+__device__ __forceinline__ void first_unknowns(const T* f, const T* ib, const T* xlp, const T* xl,
+                                               T* xf) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int q = v < 3 ? 0 : 1;
+        xf[v] = (f[6 + v] - f[3 * q] * xlp[v] - f[3 * q + 2] * xl[v]) * ib[q];
+    }
+}
+
+// Reduced system of one line held in registers by RP consecutive lanes (a power
+// of two; lane i of the group holds chunk i's rows f, l; R chunks, identity rows
+// beyond): compaction to the last unknowns, PCR across the lanes, then the first
+// unknowns.  Every lane of the wavefront must call it.
+template <class T>
+__device__ __forceinline__ void solve_chunks_in_lanes(const T* f, const T* l, int i, int R, int RP,
+                                                      T* xf, T* xl) {
+    const int lane = threadIdx.x & 63;
+    const bool hn = i + 1 < R;
+    T fn[kRedFields];
+#pragma unroll
+    for (int k = 0; k < kRedFields; k++) {
+        const T v = __shfl(f[k], hn ? lane + 1 : lane, 64);
+        fn[k] = hn ? v : ((k == 1 || k == 4) ? (T)1 : (T)0);
+    }
+    T A[2], B[2], C[2], D[4], ib[2];
+    compact_rows<T>(f, l, fn, A, B, C, D, ib);
+    pcr_solve<T>(A, B, C, D, i, R, RP);
+    T xlp[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const T t = __shfl(D[v], i > 0 ? lane - 1 : lane, 64);
+        xlp[v] = i > 0 ? t : (T)0;
+        xl[v] = D[v];
+    }
+    first_unknowns<T>(f, ib, xlp, xl, xf);
 }
 
 // Reduced systems staged in LDS as [line][field][row] with a line pitch of
@@ -197,33 +279,50 @@ struct RedLayout {
     }
 };
 
-// Solves `nl` reduced systems held in LDS with the lanes of `nw` wavefronts
-// (this one is wave `w`); writes x over the D fields.
+// Solves `nl` reduced systems held in LDS (row 2i = f_i, row 2i+1 = l_i of
+// chunk i, L.R / 2 chunks per line) with the lanes of `nw` wavefronts (this one
+// is wave `w`): compaction to one unknown per chunk, one lane per chunk with the
+// lanes of a line inside one wavefront, PCR over L.R / 2 rows; writes x_f and
+// x_l over the D fields of rows 2i and 2i+1.
 template <class T>
-__device__ __forceinline__ void solve_reduced_lines(T* red, RedLayout L, int nl, int w, int nw) {
+__device__ __forceinline__ void solve_reduced_compact(T* red, RedLayout L, int nl, int w, int nw) {
     const int lane = threadIdx.x & 63;
+    const int P = L.R / 2;
     int RP = 1;
-    while (RP < L.R) RP <<= 1;
-    const int lpp = 64 / RP;                 // lines per pass of one wave
+    while (RP < P) RP <<= 1;
+    const int lpp = 64 / RP;  // lines per pass of one wave
     const int i = lane % RP;
     for (int base = w * lpp; base < nl; base += nw * lpp) {
         const int line = base + lane / RP;
-        const int Rl = line < nl ? L.R : 0;  // rows of this lane's system (0: no system)
-        const bool act = i < Rl;
-        T A[2], B[2], C[2], D[4];
-        if (act) {
-            A[0] = red[L.at(line, 0, i)]; B[0] = red[L.at(line, 1, i)]; C[0] = red[L.at(line, 2, i)];
-            A[1] = red[L.at(line, 3, i)]; B[1] = red[L.at(line, 4, i)]; C[1] = red[L.at(line, 5, i)];
+        const int Rl = line < nl ? P : 0;
+        const bool act = i < Rl, actn = i + 1 < Rl;
+        T f[kRedFields], l[kRedFields], fn[kRedFields];
 #pragma unroll
-            for (int v = 0; v < 4; v++) D[v] = red[L.at(line, 6 + v, i)];
-        } else {
-            A[0] = A[1] = C[0] = C[1] = 0; B[0] = B[1] = 1;
-            D[0] = D[1] = D[2] = D[3] = 0;
+        for (int k = 0; k < kRedFields; k++) {
+            const T id = (k == 1 || k == 4) ? (T)1 : (T)0;
+            f[k] = act ? red[L.at(line, k, 2 * i)] : id;
+            l[k] = act ? red[L.at(line, k, 2 * i + 1)] : id;
+            fn[k] = actn ? red[L.at(line, k, 2 * i + 2)] : id;
         }
+        T A[2], B[2], C[2], D[4], ib[2];
+        compact_rows<T>(f, l, fn, A, B, C, D, ib);
         pcr_solve<T>(A, B, C, D, i, Rl, RP);
+        T xlp[4], xl[4], xf[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const T t = __shfl(D[v], i > 0 ? lane - 1 : lane, 64);
+            xlp[v] = i > 0 ? t : (T)0;
+            xl[v] = D[v];
+        }
+        first_unknowns<T>(f, ib, xlp, xl, xf);
+        // the reads of row 2i+2 above (lane i) precede this write (lane i+1,
+        // same wavefront) in program order
         if (act) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) red[L.at(line, 6 + v, i)] = D[v];
+            for (int v = 0; v < 4; v++) {
+                red[L.at(line, 6 + v, 2 * i)] = xf[v];
+                red[L.at(line, 6 + v, 2 * i + 1)] = xl[v];
+            }
         }
     }
 }
