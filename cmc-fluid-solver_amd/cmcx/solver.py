@@ -384,12 +384,12 @@ def make_solver(dom, dtype: str = "f32", path: int = _lib.CMCX_PATH_PLAIN,
                 merge_order: int = _lib.CMCX_MERGE_CPU, params: FluidParams | None = None,
                 nblockz: int = 1, variant: int | None = None) -> AdiSolver3D:
     """Init + SetOptions + CreateSegments for a domain in one call; `variant`
-    forces the fused path's X/Y tile variant."""
+    forces the fused path's X/Y tile variant (the Z sweep takes min(variant, 7))."""
     s = AdiSolver3D(dtype)
     s.Init(GPU, False, dom, params or dom.params(), nblockZ=nblockz)
     s.set_options(path, merge_order)
     s.CreateSegments()
     if variant is not None and path == _lib.CMCX_PATH_FUSED:
         for axis in range(3):
-            s.set_tile_variant(axis, min(variant, 1) if axis == Z else variant)
+            s.set_tile_variant(axis, min(variant, 7) if axis == Z else variant)
     return s

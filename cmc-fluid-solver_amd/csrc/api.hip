@@ -221,9 +221,20 @@ class Solver final : public ISolver {
             // TimeLayer3D(backend, grid) -> CopyFromGrid (TimeLayer3D.h:734-781)
             for (int f = 0; f < 4; f++)
                 CMCX_HIP(hipMemcpyAsync(cur.f[f], d_g[f], sizeof(T) * NT, hipMemcpyDeviceToDevice, st));
+            copy_state_to_all(0, NT);
             CMCX_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
         }
         upload_desc();
+    }
+
+    // The reference allocates next, half and temp without initialising them
+    // (TimeLayer3D.h:1065-1075); here every layer starts as a copy of the state,
+    // and so after set_fields.  Nodes that no segment covers then hold the same
+    // value in cur and next, which lets the fused Z sweep write its input there
+    // (a whole-vector store) without changing any result.
+    void copy_state_to_all(int64_t off, int64_t n) {
+        const Layer4<T> src = shifted(cur, off);
+        for (const Layer4<T>* L : {&next, &half, &tA, &tB}) launch_copy4<T>(shifted(*L, off), src, n, st);
     }
 
     // Descriptor table of the fused kernels (dt-independent part of SweepParams).
@@ -501,8 +512,8 @@ class Solver final : public ISolver {
         } else {
             const int v = variant[axis];
             const int lv = axis == AX_Z ? 0 : v;
-            launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_type, d_bcv, d_bct, d_g,
-                                  in, tin, out, tout, merges, v, st);
+            launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_type, d_bcv, d_bct,
+                                  d_g, in, tin, out, tout, merges, v, st);
         }
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
@@ -682,6 +693,7 @@ class Solver final : public ISolver {
             if (src[f])
                 CMCX_HIP(hipMemcpyAsync(cur.f[f] + Gr.off, src[f], sizeof(T) * Gr.g.N,
                                         hipMemcpyHostToDevice, st));
+        copy_state_to_all(Gr.off, Gr.g.N);
         CMCX_HIP(hipStreamSynchronize(st));
         return CMCX_OK;
     }
@@ -968,6 +980,13 @@ class Solver final : public ISolver {
                 const int k = 3 * it + q;
                 Layer4<T> in = q == 0 ? S : lay(sb, 1 + 3 * G + k - 1);
                 Layer4<T> out = lay(sb, 1 + 3 * G + k), tout = lay(sb, k + 1);
+                // the step's Z and X sweeps share one output buffer, which holds
+                // the state off the segments at the start of the step: the last
+                // Z sweep starts from the state and the last X sweep from the Z
+                // sweep's output, whose values they keep on the nodes they do
+                // not cover (the X output becomes the new state)
+                if (q == 0 && it == G - 1) copy_layer(out, S);
+                if (q == 2 && it == G - 1) copy_layer(out, lay(sb, 1 + 3 * G + 3 * it));
                 sweep(axes[q], in, tin, out, tout, q == 2 ? 2 : 1, 0, 4, dt);
                 tin = tout;
             }

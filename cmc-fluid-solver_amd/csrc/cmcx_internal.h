@@ -88,10 +88,11 @@ void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const Swee
 // 2 = 32-byte rows, each with chunks of 8 rows per lane; 3-5 the same with
 // chunks of 4 rows, for lines of at most 128 nodes; 6-9 = variants 1, 2, 4, 5
 // with the Picard layer kept in LDS for the merge; strided_lpg gives the z
-// lines of a tile) and, for the Z sweep, the register cap (0 = compiler's
-// choice, 1 = capped for more waves per SIMD).
-constexpr int kFusedVariants = 10;  // strided sweeps; the Z sweep has 2 (0, 1)
-constexpr int kZVariants = 2;
+// lines of a tile) and, for the Z sweep, bit 0 = registers capped for more
+// waves per SIMD (else the compiler's choice), bit 1 = the Picard layer parked
+// in LDS until the merge, bit 2 = a line spread over 64 lanes (else 32).
+constexpr int kFusedVariants = 10;  // strided sweeps
+constexpr int kZVariants = 8;
 constexpr int kZWaves = 4;  // wavefronts per Z-sweep work-group (two lines each)
 
 // Descriptor of one grid of a batch.  three_dt is passed per launch.

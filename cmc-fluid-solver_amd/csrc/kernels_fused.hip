@@ -10,11 +10,20 @@
 //                           systems hold x = node value there (no-slip), which
 //                           decouples the two sides (a free shared node is
 //                           routed to the per-segment kernel by the host)
-//   otherwise               identity row x = 0, never written
+//   otherwise               identity row x = 0, not stored (Z: see "Stores")
 // On a well-formed mask the line-wide matrix is block diagonal with exactly the
 // reference's segments (Grid3D.cpp:62-118) as blocks, so solving it gives every
 // segment's Thomas solution up to rounding.  Outputs follow UpdateSegment
 // (every segment node written) and the Picard merge on NODE_IN nodes.
+//
+// Stores: the Z sweep writes every row of a line with whole-vector stores (a
+// chunk stored with gaps costs far more than its bytes); rows no segment covers
+// receive the sweep input's value.  That is what the reference leaves in its
+// output `next` at the end of the step: every layer starts as the state, the
+// step set-up copies BOUND and VALVE values of the state into next, and where
+// the X sweep that follows covers the node it overwrites it, here too.  The
+// strided (X, Y) sweeps store segment rows only, as the reference (storing the
+// other rows as well costs them more in bytes than it gains).
 //
 // Each line is solved by the chunked partition method of line_solve.h (chunks
 // of M rows per lane, reduced system of 2 rows per chunk, compacted by one
@@ -260,18 +269,22 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
 // vector accesses (the wave covers its two lines with back-to-back vectors);
 // the along-line neighbours of a chunk come from the adjacent lanes by shuffle
 // and the cross-line derivatives of w from vector loads of the neighbour lines.
-// The reduced system of a line never leaves its 32 lanes (no LDS, no barrier).
-template <class T, int M, int MINW>
-__global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
+// The reduced system of a line never leaves its lanes (no barrier).  PARK: the
+// lane's Picard-layer values wait for the merge in LDS (16-byte slots,
+// lane-contiguous) instead of registers.  LPL = 64: a line spans a whole
+// wavefront (half the rows per lane, one line per wave, twice the waves per
+// work-group) for fewer registers per lane.
+template <class T, int M, int MINW, bool PARK, int LPL>
+__global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
     const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
     const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
     Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
     const int32_t* __restrict__ lines, const int32_t* __restrict__ winst, int vec_ok) {
+    extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
     // every line of a work-group belongs to one instance
     const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int inst = winst[b];
@@ -279,10 +292,11 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     SweepParams<T> p = insts[inst].p[AX_Z];
     p.three_dt = three_dt;
     const int n = g.nz;
-    const int used = (n + M - 1) / M;  // lanes of a line holding rows (<= 32)
+    const int used = (n + M - 1) / M;  // lanes of a line holding rows (<= LPL)
 
-    const int sub = lane >> 5, ll = lane & 31;
-    const int lid = lines[b * (2 * nw) + 2 * wave + sub];  // x*ny + y, or -1 (padding)
+    // 2 * kZWaves lines per work-group, 64 / LPL per wavefront
+    const int sub = lane / LPL, ll = lane % LPL;
+    const int lid = lines[b * (2 * kZWaves) + (64 / LPL) * wave + sub];  // x*ny + y, or -1
     const bool live = lid >= 0;
     const int64_t base = live ? insts[inst].off + (int64_t)lid * g.nz : 0;
     const int r0 = ll * M;
@@ -299,7 +313,7 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
         int prev = __shfl((int)ty[M], lane - 1, 64);
         int next = __shfl((int)ty[1], lane + 1, 64);
         ty[0] = (ll > 0 && ok) ? (uint8_t)prev : kOutside;
-        ty[M + 1] = (ll < 31 && ok && r0 + M < n) ? (uint8_t)next : kOutside;
+        ty[M + 1] = (ll < LPL - 1 && ok && r0 + M < n) ? (uint8_t)next : kOutside;
     }
     int role[M];
 #pragma unroll
@@ -330,6 +344,22 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     for (int v = 0; v < 4; v++) {
         tprev[v] = __shfl(tv[v][M - 1], lane - 1, 64);
         tnext[v] = __shfl(tv[v][0], lane + 1, 64);
+    }
+    using V16 = typename Vec16<T>::type;
+    constexpr int VW = Vec16<T>::n;
+    constexpr int NS = 4 * ((M + VW - 1) / VW);  // 16-byte slots per lane
+    V16* park = reinterpret_cast<V16*>(smem_raw) + (int64_t)wave * NS * 64 + lane;
+    if (PARK) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int k = 0; k < (M + VW - 1) / VW; k++) {
+                V16 x;
+                T* e = reinterpret_cast<T*>(&x);
+#pragma unroll
+                for (int q = 0; q < VW; q++) e[q] = k * VW + q < M ? tv[v][k * VW + q] : (T)0;
+                park[(v * ((M + VW - 1) / VW) + k) * 64] = x;
+            }
     }
 
     Chunk<T, M> ch;
@@ -372,12 +402,23 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
     {
         T f[kRedFields], l[kRedFields];
         chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
-        solve_chunks_in_lanes<T>(f, l, ll, used, 32, xf, xl);
+        solve_chunks_in_lanes<T>(f, l, ll, used, LPL, xf, xl);
     }
     if (!ok) return;
-    bool all_written = true;
+    if (PARK) {
+        // the parked copy is read back (not the registers of the loads)
+        asm volatile("" ::: "memory");
 #pragma unroll
-    for (int j = 0; j < M; j++) all_written = all_written && role[j] != RL_NONE;
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+            for (int k = 0; k < (M + VW - 1) / VW; k++) {
+                const V16 x = park[(v * ((M + VW - 1) / VW) + k) * 64];
+                const T* e = reinterpret_cast<const T*>(&x);
+#pragma unroll
+                for (int q = 0; q < VW; q++)
+                    if (k * VW + q < M) tv[v][k * VW + q] = e[q];
+            }
+    }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         T x[M], t[M];
@@ -390,18 +431,14 @@ __global__ void __launch_bounds__(64 * kZWaves, MINW) fused_z_kernel(
                 const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
                 x[j] = ch.E[v][j] + F * xf[v] + G * xl[v];
             }
-            // the Picard layer keeps its value off NODE_IN (rewritten unchanged)
+            // rows no segment covers get the sweep input's value
+            if (role[j] == RL_NONE) x[j] = cu[v][j];
             T m = (tv[v][j] + x[j]) * (T)0.5;
             if (merges > 1) m = (m + x[j]) * (T)0.5;
             t[j] = role[j] == RL_IN ? m : tv[v][j];
         }
-        if (all_written && vec) {
-            store_run<T, M>(out.f[v] + c0, x, true, M);
-        } else {
-#pragma unroll
-            for (int j = 0; j < M; j++)
-                if (j < valid && role[j] != RL_NONE) out.f[v][c0 + j] = x[j];
-        }
+        // whole runs: partial-vector stores would cost far more than the bytes
+        store_run<T, M>(out.f[v] + c0, x, vec, valid);
         store_run<T, M>(tout.f[v] + c0, t, vec, valid);
     }
 }
@@ -431,9 +468,11 @@ int strided_waves_for(int n, int lpg, int m = kChunk) {
     return (n + m * gpw - 1) / (m * gpw);
 }
 
-int z_chunk(int n) {
+// Rows per lane of a Z line spread over `lpl` lanes (0: too long).  32-lane
+// lines take up to 16 rows per lane, 64-lane lines up to 8.
+int z_chunk(int n, int lpl = 32) {
     for (int m : {2, 4, 8, 16})
-        if (32 * m >= n) return m;
+        if (lpl * m >= n && (lpl == 32 || m <= 8)) return m;
     return 0;
 }
 
@@ -486,25 +525,42 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
     if (w.nblocks == 0) return;
     const unsigned blocks = (unsigned)w.nblocks;
     if (axis == AX_Z) {
-        const int m = z_chunk(w.maxn);
-        const size_t lds = 0;  // the reduced systems stay in registers
-        // variant 1 caps the registers for 4 (FP32) / 2 (FP64) waves per SIMD
+        // variants 1, 3, 5, 7 cap the registers for 4 (FP32) / 2 (FP64) waves per
+        // SIMD; 2, 3, 6, 7 park the Picard layer in LDS; 4-7 spread a line over
+        // a whole wavefront
         constexpr int MW = sizeof(T) == 8 ? 2 : 4;
-#define LZ(MM, W)                                                                               \
-    hipLaunchKernelGGL((fused_z_kernel<T, MM, W>), dim3(blocks), dim3(64 * kZWaves), lds, st,  \
-                       insts, three_dt, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in, \
+        const bool park = (variant & 2) != 0, wide = (variant & 4) != 0;
+        const int lpl = wide ? 64 : 32;
+        const int m = z_chunk(w.maxn, lpl);
+        if (m == 0) throw std::string("fused sweep: Z variant does not fit the line length");
+        const int vw = 16 / (int)sizeof(T);
+        const int nthreads = 64 * kZWaves * (lpl / 32);
+        const size_t lds = park ? (size_t)(nthreads / 64) * 64 * 4 * ((m + vw - 1) / vw) * 16 : 0;
+#define LZ(MM, W, PK, LL)                                                                          \
+    hipLaunchKernelGGL((fused_z_kernel<T, MM, W, PK, LL>), dim3(blocks), dim3(nthreads), lds, st, \
+                       insts, three_dt, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in,     \
                        tin, out, tout, merges, w.d_items, w.d_inst, w.vec_ok)
-        if (variant == 0) {
-            if (m == 2) LZ(2, 1);
-            else if (m == 4) LZ(4, 1);
-            else if (m == 8) LZ(8, 1);
-            else LZ(16, 1);
+#define LZM(W, PK)                      \
+    if (!wide) {                        \
+        if (m == 2) LZ(2, W, PK, 32);   \
+        else if (m == 4) LZ(4, W, PK, 32); \
+        else if (m == 8) LZ(8, W, PK, 32); \
+        else LZ(16, 1, PK, 32);         \
+    } else {                            \
+        if (m == 2) LZ(2, W, PK, 64);   \
+        else if (m == 4) LZ(4, W, PK, 64); \
+        else LZ(8, W, PK, 64);          \
+    }
+        if ((variant & 3) == 0) {
+            LZM(1, false)
+        } else if ((variant & 3) == 1) {
+            LZM(MW, false)
+        } else if ((variant & 3) == 2) {
+            LZM(1, true)
         } else {
-            if (m == 2) LZ(2, MW);
-            else if (m == 4) LZ(4, MW);
-            else if (m == 8) LZ(8, MW);
-            else LZ(16, 1);
+            LZM(MW, true)
         }
+#undef LZM
 #undef LZ
     } else {
         const int lpg = variant_lpg<T>(variant);

@@ -200,8 +200,9 @@ int cmcx_autotune(cmcx_solver* s, int reps, int* choice);
    with half the rows per lane (lines of at most 128 nodes); 6, 7, 8, 9 =
    variants 1, 2, 4, 5 keeping the Picard layer in LDS between the elimination
    and the merge (variants that do not fit the grid return
-   CMCX_ERR_UNSUPPORTED).  Z sweep (2): 0 = register
-   use left to the compiler, 1 = capped for more waves per SIMD.  No reference
+   CMCX_ERR_UNSUPPORTED).  Z sweep (2), 0-7: bit 0 = registers capped for more
+   waves per SIMD, bit 1 = the Picard layer parked in LDS until the merge,
+   bit 2 = each z line spread over 64 lanes instead of 32.  No reference
    counterpart (tuning knob). */
 int cmcx_set_tile_variant(cmcx_solver* s, int axis, int variant);
 /* FluidParams of the selected grid (e.g. a new viscosity between rollouts). */

@@ -30,6 +30,11 @@
 //       AdiSolver3D.cu:337-364, 479-486): U, V, W, T solved one after another,
 //       each merging its own field into temp right after its solve; no host merges.
 //
+// Layers other than the state: the reference allocates next, half and temp
+// without initialising them (TimeLayer3D.h:1065-1075), so their values at nodes
+// that no segment covers are unspecified.  Here every layer starts as a copy of
+// the state, and set_fields writes every layer: such nodes keep their values.
+//
 // Shared segment endpoints (a one-cell wall between two runs on one line) are
 // written by both segments in the reference, in parallel (a race when the wall
 // is BC_FREE, AdiSolver3D.cpp:593-603).  Here the solve runs in parallel into
@@ -219,7 +224,12 @@ struct Solver {
                     cur->e(VAR_W, i, j, k) = gv[2][id];
                     cur->e(VAR_T, i, j, k) = gT[id];
                 }
+        copy_state_to_all();
         max_n = std::max(nx, std::max(ny, nz));
+    }
+    void copy_state_to_all() {
+        for (Layer<T>* L : {next, half, temp})
+            for (int v = 0; v < 4; v++) L->f[v] = cur->f[v];
     }
     inline size_t idx(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
 
@@ -805,6 +815,7 @@ void orc_set_fields(void* h, const double* u, const double* v, const double* w, 
         const double* in[4] = {u, v, w, T};
         for (int c = 0; c < 4; c++)
             for (size_t i = 0; i < s->N; i++) s->cur->f[c][s->cur->halo + i] = (decltype(s->dx))in[c][i];
+        s->copy_state_to_all();
     });
 }
 

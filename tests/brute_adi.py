@@ -12,7 +12,9 @@ system whose rows are
 with w the Picard layer's velocity component along the sweep, nu = v_vis for
 u, v, w and t_vis for T, source = -v_T dT/dd for the component along the
 sweep and + t_phi Phi_d for T.  Here each system is assembled as a dense matrix
-and solved with numpy.linalg.solve.
+and solved with numpy.linalg.solve.  The layers other than the state start as
+copies of it (the reference leaves them uninitialised; the oracle and the
+product define them so).
 """
 from __future__ import annotations
 
@@ -112,10 +114,10 @@ def brute_step(dom, p, dt, G, L, state, gpu_order=False):
     cur = [a.reshape(shape).copy() for a in state]
     for k in range(4):
         cur[k][bound] = g[k][bound]            # UpdateBoundaries
-    nxt = [np.zeros(shape) for _ in range(4)]
-    half = [np.zeros(shape) for _ in range(4)]
-    for k in range(4):
-        nxt[k][bound] = cur[k][bound]
+    # every layer holds the state at the start (the step set-up also copies
+    # BOUND/VALVE values into next); nodes no segment covers keep that value
+    nxt = [a.copy() for a in cur]
+    half = [a.copy() for a in cur]
     temp = [a.copy() for a in cur]
     inside = t == 0
     segs = {a: _segments(t, a) for a in range(3)}
