@@ -205,7 +205,8 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
     """The inlet-control gradient workload: 96^3 channel with a seeded cylinder,
     loss = mean kinetic energy over a 16^3 downstream probe after 200 steps,
     gradient w.r.t. the 32x32 inlet control and nu; FP32 forward and adjoint
-    timed (adjoint = checkpointed backward including the recompute), gradients
+    timed (adjoint = checkpointed backward including the recompute; one reversed
+    step is timed as the difference of two fully taped backwards), gradients
     checked against central differences (eps 1e-3) of the FP64 forward along
     `ndir` seeded directions, and against the FP64 adjoint."""
     from cmcx import domains
@@ -263,6 +264,27 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
         out["gc"] = block_sum(gb[0].astype(np.float64))
         out["gnu"] = float(pb[1])
         s.adjoint_release()
+        if timed:
+            # the reverse steps alone: every step taped during the forward (no
+            # checkpoint recompute), backwards of `steps` and `steps // 2` steps;
+            # their difference removes the per-call part (seed upload, gradient
+            # download), leaving the cost of one reversed step
+            def taped_backward(n):
+                s.set_fields(*st0)
+                s.adjoint_forward(n, d.dt, G, n)
+                s.synchronize()
+                _sync()
+                t0 = time.perf_counter()
+                s.adjoint_backward(seed)
+                _sync()
+                t = time.perf_counter() - t0
+                s.adjoint_release()
+                return t
+
+            half_n = steps // 2
+            t_full, t_half = taped_backward(steps), taped_backward(half_n)
+            out["reverse_step_s"] = (t_full - t_half) / (steps - half_n)
+            out["reverse_fixed_s"] = t_full - steps * out["reverse_step_s"]
 
         def L(c, nu):
             p = d.params()
@@ -306,6 +328,14 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
         "recorded_forward_ms": r32["recorded_forward_s"] * 1e3,
         "backward_ms": r32["backward_s"] * 1e3,
         "adjoint_forward_ratio": r32["backward_s"] / r32["forward_s"],
+        "reverse_step_ms": r32["reverse_step_s"] * 1e3,
+        "forward_step_ms": r32["forward_s"] / steps * 1e3,
+        "adjoint_step_ratio": r32["reverse_step_s"] / (r32["forward_s"] / steps),
+        "backward_call_fixed_ms": r32["reverse_fixed_s"] * 1e3,
+        "ratio_note": "adjoint_forward_ratio = checkpointed backward (every 20, recompute and "
+                      "host transfers included) / forward; adjoint_step_ratio = one reversed step "
+                      "(all steps taped; difference of backwards over 200 and 100 steps) / one "
+                      "forward step",
         "loss_fp32": r32["loss"],
         "loss_fp64": r64["loss"],
         "fd_directions": ndir,
