@@ -1,0 +1,125 @@
+"""GPU edge cases through the C ABI against the oracle: a domain with no NODE_IN
+node (empty segment lists and work lists), the longest lines each fused kernel
+takes (Z lines of 512 nodes in FP32 -- 16 rows per lane, or 8 on 64 lanes --
+and of 256 in FP64; X and Y lines of 256), and the rejection of longer lines
+with CMCX_ERR_UNSUPPORTED."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from cmcx import domains
+from cmcx._lib import CmcxError
+from cmcx.solver import make_solver
+from gpu_util import rel_l2, rel_l2_scaled, say, vel_norm
+
+pytestmark = pytest.mark.gpu
+
+PLAIN, FUSED = 0, 1
+
+
+def _no_fluid(shape, seed=0):
+    """A box whose every node is a wall (NODE_BOUND), with random node values."""
+    rng = np.random.default_rng(seed)
+    d = domains.box(*shape)
+    t = d.type.reshape(shape)
+    t[t == domains.NODE_IN] = domains.NODE_BOUND
+    d.vx = rng.normal(size=d.n)
+    d.vy = rng.normal(size=d.n)
+    d.vz = rng.normal(size=d.n)
+    d.T = 1.0 + 0.1 * rng.normal(size=d.n)
+    return d
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
+def test_no_fluid_node(path, dtype):
+    """No segment anywhere: the step only carries the boundary fill through, and
+    EvalDivError divides 0 by 0 like the reference (TimeLayer3D.h:595-641),
+    which its guard (err > 0.01) lets pass."""
+    d = _no_fluid((9, 7, 12))
+    say(f"[no fluid] path={path} {dtype}")
+    ref = oracle_lib.Oracle3D(d, dtype)
+    ref.create_segments()
+    s = make_solver(d, dtype, path=path)
+    for _ in range(2):
+        ref.update_boundaries()
+        s.UpdateBoundaries()
+        r, e_ref = ref.step(d.dt, 2, 1, True)
+        assert r == 0
+        e = s.TimeStep(d.dt, 2, 1, True)
+        assert np.isnan(e) and np.isnan(e_ref)
+    got = s.fields()
+    want = ref.fields()
+    for k, idx in enumerate((0, 1, 2, 4)):
+        assert np.array_equal(got[idx], want[k]), k
+    assert np.array_equal(got[0], d.vx.astype(got[0].dtype))
+    s.close()
+
+
+def _long(shape, seed):
+    """Random solids and a random initial state on the NODE_IN nodes."""
+    rng = np.random.default_rng(seed)
+    d = domains.random_domain(shape, rng, 0.12)
+    inner = d.type == domains.NODE_IN
+    for a in (d.vx, d.vy, d.vz):
+        a[inner] = rng.normal(0.0, 0.3, int(inner.sum()))
+    d.T[inner] = 1.0 + 0.1 * rng.normal(size=int(inner.sum()))
+    return d
+
+
+def _compare(d, dtype, path, variants, steps=2, tol=None):
+    ref = oracle_lib.Oracle3D(d, "f64")
+    ref.create_segments()
+    for _ in range(steps):
+        ref.update_boundaries()
+        r, _ = ref.step(d.dt, 2, 1, False)
+        assert r == 0
+    want = ref.fields()
+    vs = vel_norm(*want[:3])
+    tol = tol or (1e-11 if dtype == "f64" else 1e-5)
+    for v in variants:
+        s = make_solver(d, dtype, path=path)
+        for a, va in enumerate(v):
+            s.set_tile_variant(a, va)
+        for _ in range(steps):
+            s.UpdateBoundaries()
+            s.TimeStep(d.dt, 2, 1, False)
+        got = s.fields()
+        for k in range(3):
+            e = rel_l2_scaled(got[k], want[k], vs)
+            say(f"  {d.shape} {dtype} variants {v} field {k}: {e:.3e}")
+            assert e < tol, (v, k)
+        e = rel_l2(got[4], want[3])
+        assert e < tol, (v, "T")
+        s.close()
+
+
+@pytest.mark.parametrize("dtype,nz", [("f32", 512), ("f64", 256)])
+def test_longest_z_lines(dtype, nz):
+    """Every Z variant at the longest z lines the fused path takes."""
+    d = _long((10, 9, nz), 3)
+    say(f"[long z] {dtype} nz={nz}")
+    _compare(d, dtype, FUSED, [(0, 0, z) for z in range(8)])
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_longest_strided_lines(dtype):
+    """X and Y lines of 256 nodes: the 128-, 64- and 32-byte tile variants."""
+    for shape in ((256, 9, 12), (9, 256, 12)):
+        d = _long(shape, 5)
+        say(f"[long strided] {dtype} {shape}")
+        _compare(d, dtype, FUSED, [(v, v, 0) for v in (0, 1, 2)])
+
+
+@pytest.mark.parametrize("dtype,shape", [("f32", (8, 8, 1024)), ("f64", (8, 8, 512)),
+                                         ("f32", (512, 8, 8)), ("f64", (8, 512, 8))])
+def test_longer_lines_rejected(dtype, shape):
+    """Lines beyond the fused kernels' reach: the fused path is refused with
+    CMCX_ERR_UNSUPPORTED (the plain path takes them)."""
+    d = _long(shape, 7)
+    with pytest.raises(CmcxError):
+        make_solver(d, dtype, path=FUSED)
+    s = make_solver(d, dtype, path=PLAIN)
+    s.UpdateBoundaries()
+    s.TimeStep(d.dt, 1, 1, False)
+    s.close()
