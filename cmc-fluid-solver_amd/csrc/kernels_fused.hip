@@ -318,6 +318,26 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     int role[M];
 #pragma unroll
     for (int j = 0; j < M; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
+    // boundary rows (ApplyBC0/1): node values and BC kinds of the whole chunk in
+    // one vector load per array, for lanes whose chunk has such a row (a load
+    // per boundary row would sit inside the elimination chain)
+    bool anyb = false;
+#pragma unroll
+    for (int j = 0; j < M; j++) anyb = anyb || (role[j] != RL_IN && role[j] != RL_NONE);
+    T gb[4][M];
+    uint8_t kv[M], kt[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        kv[j] = kt[j] = 0;
+#pragma unroll
+        for (int v = 0; v < 4; v++) gb[v][j] = 0;
+    }
+    if (anyb) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) load_run<T, M>(gv[v] + c0, gb[v], vec, valid);
+        load_bytes<M>(bcv + c0, kv, vec, valid);
+        load_bytes<M>(bct + c0, kt, vec, valid);
+    }
 
     // inputs: state (in), Picard layer (tin) and the cross-line derivatives of w
     T cu[4][M], tv[4][M], wx[M], wy[M];
@@ -389,7 +409,12 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
             const T phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * wx[j] + v_z * wy[j];
             r.d[3] = r.d[3] + p.t_phi * phi;
         } else {
-            boundary_row<T>(role[j], c0 + j, bcv, bct, gv, r);
+            bool uv, ut;
+            bc_row<T>(role[j], role[j] != RL_NONE && kv[j] == BC_FREE, r.av, r.bv, r.cv, uv);
+            bc_row<T>(role[j], role[j] != RL_NONE && kt[j] == BC_FREE, r.at, r.bt, r.ct, ut);
+#pragma unroll
+            for (int v = 0; v < 3; v++) r.d[v] = uv ? gb[v][j] : (T)0;
+            r.d[3] = ut ? gb[3][j] : (T)0;
         }
         if (j == 0) rfirst = r;
         else if (j == M - 1) rlast = r;

@@ -20,6 +20,8 @@
 
 #include "cmcx_internal.h"
 
+#include <type_traits>
+
 namespace cmcx {
 namespace lsolve {
 
@@ -361,6 +363,30 @@ __device__ __forceinline__ void load_run(const T* __restrict__ p, T* r, bool vec
     }
 #pragma unroll
     for (int j = 0; j < M; j++) r[j] = j < valid ? p[j] : (T)0;
+}
+
+// r[0..M) <- bytes p[0..M): one M-byte load when `vec` and p is M-byte aligned
+// (M = 2, 4, 8, 16), else byte loads of the first `valid` bytes (zero beyond).
+template <int M>
+__device__ __forceinline__ void load_bytes(const uint8_t* __restrict__ p, uint8_t* r, bool vec,
+                                           int valid) {
+    if (vec && ((uintptr_t)p % M) == 0) {
+        if constexpr (M == 2) {
+            const uint16_t w = *reinterpret_cast<const uint16_t*>(p);
+            r[0] = (uint8_t)w; r[1] = (uint8_t)(w >> 8);
+            return;
+        } else if constexpr (M == 4 || M == 8 || M == 16) {
+            using W = typename std::conditional<M == 4, uint32_t,
+                      typename std::conditional<M == 8, uint2, uint4>::type>::type;
+            const W w = *reinterpret_cast<const W*>(p);
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(&w);
+#pragma unroll
+            for (int j = 0; j < M; j++) r[j] = b[j];
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) r[j] = j < valid ? p[j] : (uint8_t)0;
 }
 
 template <class T, int M>
