@@ -104,6 +104,11 @@ class Solver final : public ISolver {
 
     std::vector<uint8_t> h_type, h_bcv, h_bct;  // host copies of the flat mask / BC kinds
     uint8_t *d_type = nullptr, *d_bcv = nullptr, *d_bct = nullptr;
+    uint8_t* d_tflags = nullptr;  // node flags of the fused kernels (build_node_flags)
+    // the state holds the node values on BOUND/VALVE nodes with no-slip BCs:
+    // set by UpdateBoundaries (and at creation, the state being the node values),
+    // kept by steps, cleared when the caller replaces the state or node values
+    bool bv_synced = true;
     T* d_g[4] = {nullptr, nullptr, nullptr, nullptr};
     T* buf = nullptr;  // 5 layers x 4 fields x NT
     Layer4<T> cur{}, next{}, half{}, tA{}, tB{};
@@ -120,7 +125,8 @@ class Solver final : public ISolver {
     int64_t n_nonin = 0, n_bv = 0;
     // HIP graphs of whole steps, per (boundary fill, dt, G, L, state buffer)
     bool graph_mode = false;
-    std::map<std::tuple<int, double, int, int, const void*>, hipGraphExec_t> graphs;
+    // (with the boundary-sync mode the sweeps were captured with)
+    std::map<std::tuple<int, double, int, int, const void*, int>, hipGraphExec_t> graphs;
     hipStream_t cap = nullptr;
     // reverse mode over a recorded rollout
     struct Adj {
@@ -185,6 +191,7 @@ class Solver final : public ISolver {
         CMCX_HIP(hipMalloc((void**)&d_type, NT));
         CMCX_HIP(hipMalloc((void**)&d_bcv, NT));
         CMCX_HIP(hipMalloc((void**)&d_bct, NT));
+        CMCX_HIP(hipMalloc((void**)&d_tflags, NT));
         for (int f = 0; f < 4; f++) CMCX_HIP(hipMalloc((void**)&d_g[f], sizeof(T) * NT));
         CMCX_HIP(hipMalloc((void**)&buf, sizeof(T) * NT * 20));
         CMCX_HIP(hipMemsetAsync(buf, 0, sizeof(T) * NT * 20, st));
@@ -197,6 +204,7 @@ class Solver final : public ISolver {
         CMCX_HIP(hipMemcpyAsync(d_type, h_type.data(), NT, hipMemcpyHostToDevice, st));
         CMCX_HIP(hipMemcpyAsync(d_bcv, h_bcv.data(), NT, hipMemcpyHostToDevice, st));
         CMCX_HIP(hipMemcpyAsync(d_bct, h_bct.data(), NT, hipMemcpyHostToDevice, st));
+        upload_flags(0, NT);
         for (int f = 0; f < 4; f++)
             CMCX_HIP(hipMemcpyAsync(d_g[f], hg[f].data(), sizeof(T) * NT, hipMemcpyHostToDevice, st));
         {
@@ -225,6 +233,13 @@ class Solver final : public ISolver {
             CMCX_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
         }
         upload_desc();
+    }
+
+    void upload_flags(int64_t off, int64_t n) {
+        std::vector<uint8_t> h(n);
+        build_node_flags(&h_type[off], &h_bcv[off], &h_bct[off], n, h.data());
+        CMCX_HIP(hipMemcpyAsync(d_tflags + off, h.data(), n, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));  // h goes out of scope
     }
 
     // The reference allocates next, half and temp without initialising them
@@ -256,6 +271,7 @@ class Solver final : public ISolver {
         hipFree(d_type);
         hipFree(d_bcv);
         hipFree(d_bct);
+        hipFree(d_tflags);
         for (auto* p : d_g) hipFree(p);
         hipFree(buf);
         for (auto& G : grids)
@@ -446,6 +462,7 @@ class Solver final : public ISolver {
 
     int update_boundaries() override {
         launch_update_boundaries_list<T>(n_bv, d_bvlist, d_g, cur, st);
+        bv_synced = true;
         return CMCX_OK;
     }
 
@@ -512,8 +529,8 @@ class Solver final : public ISolver {
         } else {
             const int v = variant[axis];
             const int lv = axis == AX_Z ? 0 : v;
-            launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_type, d_bcv, d_bct,
-                                  d_g, in, tin, out, tout, merges, v, st);
+            launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_tflags, d_g, in, tin,
+                                  out, tout, merges, bv_synced ? 1 : 0, v, st);
         }
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
@@ -559,7 +576,9 @@ class Solver final : public ISolver {
     // The graph of (boundary fill +) one step for the current state buffer;
     // captured on a private stream the first time, replayed on the solver's stream.
     hipGraphExec_t step_graph(bool ub, T dt, int G, int L) {
-        auto key = std::make_tuple(ub ? 1 : 0, (double)dt, G, L, (const void*)cur.f[0]);
+        if (ub) bv_synced = true;  // the graph's boundary fill precedes its sweeps
+        auto key = std::make_tuple(ub ? 1 : 0, (double)dt, G, L, (const void*)cur.f[0],
+                                   bv_synced ? 1 : 0);
         auto it = graphs.find(key);
         if (it != graphs.end()) return it->second;
         if (!cap) CMCX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -694,12 +713,14 @@ class Solver final : public ISolver {
                 CMCX_HIP(hipMemcpyAsync(cur.f[f] + Gr.off, src[f], sizeof(T) * Gr.g.N,
                                         hipMemcpyHostToDevice, st));
         copy_state_to_all(Gr.off, Gr.g.N);
+        bv_synced = false;
         CMCX_HIP(hipStreamSynchronize(st));
         return CMCX_OK;
     }
 
     int state_ptrs(void** p4) override {
         if (!p4) return CMCX_ERR_ARG;
+        bv_synced = false;  // the caller may write the state
         for (int f = 0; f < 4; f++) p4[f] = cur.f[f] + G().off;
         return CMCX_OK;
     }
@@ -758,6 +779,8 @@ class Solver final : public ISolver {
             CMCX_HIP(hipMemcpyAsync(d_g[f] + o, tmp.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
             CMCX_HIP(hipStreamSynchronize(st));  // tmp is reused
         }
+        upload_flags(o, N);
+        bv_synced = false;
         if (built) {
             Gr.free_shared = shared_free_axes(&h_type[o], &h_bcv[o], &h_bct[o], Gr.g);
             prepare_path_buffers();
@@ -801,8 +824,8 @@ class Solver final : public ISolver {
                     const int lv = a == AX_Z ? 0 : v;  // the Z work list does not depend on v
                     ensure_work(a, lv);
                     auto launch = [&]() {
-                        launch_fused_sweep<T>(d_desc, (T)3 / dt, a, work[a][lv], d_type, d_bcv, d_bct,
-                                              d_g, cur, cur, half, tA, 1, v, st);
+                        launch_fused_sweep<T>(d_desc, (T)3 / dt, a, work[a][lv], d_tflags, d_g,
+                                              cur, cur, half, tA, 1, bv_synced ? 1 : 0, v, st);
                     };
                     launch();  // warm-up
                     CMCX_HIP(hipEventRecord(e0, st));

@@ -31,6 +31,13 @@
 // PCR across the lanes: through LDS for strided lines, in registers for Z).  The U, V and W systems share one matrix (same
 // bc_vel and viscosity); T has its own.
 //
+// Node flags: the fused kernels read one byte per node, the node type in bits
+// 0-1 and the BC kinds (BC_FREE) of velocity and T in bits 2 and 3 (0 for
+// NODE_IN).  Dirichlet values of boundary rows come from the Picard layer
+// while the state is in sync with the node values (after UpdateBoundaries;
+// steps keep it: their no-slip rows write the node values), else from the
+// node-value arrays (`bsync`).
+//
 // Layouts:
 //   strided lines (X and Y sweeps): a work-group owns LPG consecutive z lines
 //     (LPG * sizeof(T) = 128 bytes: every row access of a lane group is one
@@ -104,15 +111,17 @@ __device__ __forceinline__ void interior_row(const Geom& g, const SweepParams<T>
     r.d[3] = r.d[3] + p.t_phi * phi;
 }
 
+// Boundary or identity row (ApplyBC0/ApplyBC1) from the node's flags byte (BC
+// kinds in bits 2 and 3); a Dirichlet row takes its value from src[v][c].
 template <class T>
-__device__ __forceinline__ void boundary_row(int role, int64_t c, const uint8_t* bcv,
-                                             const uint8_t* bct, const T* const* gv, Row<T>& r) {
+__device__ __forceinline__ void boundary_row(int role, uint8_t tf, int64_t c, const T* const* src,
+                                             Row<T>& r) {
     bool uv, ut;
-    bc_row<T>(role, role != RL_NONE && bcv[c] == BC_FREE, r.av, r.bv, r.cv, uv);
-    bc_row<T>(role, role != RL_NONE && bct[c] == BC_FREE, r.at, r.bt, r.ct, ut);
+    bc_row<T>(role, role != RL_NONE && (tf & kFlagFreeVel), r.av, r.bv, r.cv, uv);
+    bc_row<T>(role, role != RL_NONE && (tf & kFlagFreeT), r.at, r.bt, r.ct, ut);
 #pragma unroll
-    for (int v = 0; v < 3; v++) r.d[v] = uv ? gv[v][c] : (T)0;
-    r.d[3] = ut ? gv[3][c] : (T)0;
+    for (int v = 0; v < 3; v++) r.d[v] = uv ? src[v][c] : (T)0;
+    r.d[3] = ut ? src[3][c] : (T)0;
 }
 
 // Loads the chunk's rows, eliminates them and returns the two reduced rows.
@@ -120,9 +129,7 @@ __device__ __forceinline__ void boundary_row(int role, int64_t c, const uint8_t*
 // identity rows.  role[] receives the row roles.
 template <class T, int AXIS, int M, bool KEEP>
 __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>& p,
-                                            const uint8_t* __restrict__ type,
-                                            const uint8_t* __restrict__ bcv,
-                                            const uint8_t* __restrict__ bct, const T* const* gv,
+                                            const uint8_t* __restrict__ type, const T* const* bsrc,
                                             const Layer4<T>& in, const Layer4<T>& tin,
                                             int64_t base, int64_t s, int r0, int n, bool ok,
                                             int* role, Chunk<T, M>& ch, T* f, T* l, T* keep,
@@ -148,7 +155,7 @@ __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>&
                 for (int v = 0; v < 4; v++) keep[v * kfs + j * krs] = tin.f[v][c];
             }
         } else {
-            boundary_row<T>(role[j], c, bcv, bct, gv, rw);
+            boundary_row<T>(role[j], ty[j + 1], c, bsrc, rw);
         }
         if (j == 0) rfirst = rw;
         else if (j == M - 1) rlast = rw;
@@ -203,13 +210,20 @@ __host__ __device__ constexpr int keep_pitch(int lpg, int m) { return lpg + (lpg
 template <class T, int AXIS, int M, int LPG, int MAXT, int MINW, bool KEEP>
 __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
-    const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
-    const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
-    Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-    const int32_t* __restrict__ work) {
+    const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
+    const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
+    int merges, int bsync, const int32_t* __restrict__ work) {
     extern __shared__ char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);
-    const T* gv[4] = {gv0, gv1, gv2, gv3};
+    // Dirichlet values of the boundary rows: the Picard layer while the state is
+    // in sync with the node values (it holds them off NODE_IN), else the node values
+    const T* bsrc[4];
+    if (bsync) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) bsrc[v] = tin.f[v];
+    } else {
+        bsrc[0] = gv0; bsrc[1] = gv1; bsrc[2] = gv2; bsrc[3] = gv3;
+    }
     constexpr int GPW = 64 / LPG;  // chunks (lane groups) per wave
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -244,8 +258,8 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     int role[M];
     Chunk<T, M> ch;
     T f[kRedFields], l[kRedFields];
-    chunk_phase<T, AXIS, M, KEEP>(g, p, type, bcv, bct, gv, in, tin, base, s, r0, n, zok, role, ch,
-                                  f, l, keep, kfs, krs);
+    chunk_phase<T, AXIS, M, KEEP>(g, p, type, bsrc, in, tin, base, s, r0, n, zok, role, ch, f, l,
+                                  keep, kfs, krs);
 #pragma unroll
     for (int k = 0; k < kRedFields; k++) {
         red[L.at(gl, k, 2 * chunk)] = f[k];
@@ -277,10 +291,10 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
 template <class T, int M, int MINW, bool PARK, int LPL>
 __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
-    const uint8_t* __restrict__ bcv, const uint8_t* __restrict__ bct, const T* __restrict__ gv0,
-    const T* __restrict__ gv1, const T* __restrict__ gv2, const T* __restrict__ gv3,
-    Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-    const int32_t* __restrict__ lines, const int32_t* __restrict__ winst, int vec_ok) {
+    const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
+    const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
+    int merges, int bsync, const int32_t* __restrict__ lines, const int32_t* __restrict__ winst,
+    int vec_ok) {
     extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
@@ -318,25 +332,24 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     int role[M];
 #pragma unroll
     for (int j = 0; j < M; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
-    // boundary rows (ApplyBC0/1): node values and BC kinds of the whole chunk in
-    // one vector load per array, for lanes whose chunk has such a row (a load
-    // per boundary row would sit inside the elimination chain)
-    bool anyb = false;
-#pragma unroll
-    for (int j = 0; j < M; j++) anyb = anyb || (role[j] != RL_IN && role[j] != RL_NONE);
+    // Dirichlet values of the boundary rows (ApplyBC0/1; the BC kinds are in
+    // the flags bytes): the Picard layer while the state is in sync with the
+    // node values (it holds them off NODE_IN), else the node values of the
+    // whole chunk in one vector load per field, for lanes whose chunk has such
+    // a row (a load per boundary row would sit inside the elimination chain)
     T gb[4][M];
-    uint8_t kv[M], kt[M];
 #pragma unroll
-    for (int j = 0; j < M; j++) {
-        kv[j] = kt[j] = 0;
+    for (int j = 0; j < M; j++)
 #pragma unroll
         for (int v = 0; v < 4; v++) gb[v][j] = 0;
-    }
-    if (anyb) {
+    if (!bsync) {
+        bool anyb = false;
 #pragma unroll
-        for (int v = 0; v < 4; v++) load_run<T, M>(gv[v] + c0, gb[v], vec, valid);
-        load_bytes<M>(bcv + c0, kv, vec, valid);
-        load_bytes<M>(bct + c0, kt, vec, valid);
+        for (int j = 0; j < M; j++) anyb = anyb || (role[j] != RL_IN && role[j] != RL_NONE);
+        if (anyb) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) load_run<T, M>(gv[v] + c0, gb[v], vec, valid);
+        }
     }
 
     // inputs: state (in), Picard layer (tin) and the cross-line derivatives of w
@@ -409,12 +422,13 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
             const T phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * wx[j] + v_z * wy[j];
             r.d[3] = r.d[3] + p.t_phi * phi;
         } else {
+            const uint8_t tf = ty[j + 1];
             bool uv, ut;
-            bc_row<T>(role[j], role[j] != RL_NONE && kv[j] == BC_FREE, r.av, r.bv, r.cv, uv);
-            bc_row<T>(role[j], role[j] != RL_NONE && kt[j] == BC_FREE, r.at, r.bt, r.ct, ut);
+            bc_row<T>(role[j], role[j] != RL_NONE && (tf & kFlagFreeVel), r.av, r.bv, r.cv, uv);
+            bc_row<T>(role[j], role[j] != RL_NONE && (tf & kFlagFreeT), r.at, r.bt, r.ct, ut);
 #pragma unroll
-            for (int v = 0; v < 3; v++) r.d[v] = uv ? gb[v][j] : (T)0;
-            r.d[3] = ut ? gb[3][j] : (T)0;
+            for (int v = 0; v < 3; v++) r.d[v] = uv ? (bsync ? tv[v][j] : gb[v][j]) : (T)0;
+            r.d[3] = ut ? (bsync ? tv[3][j] : gb[3][j]) : (T)0;
         }
         if (j == 0) rfirst = r;
         else if (j == M - 1) rlast = r;
@@ -544,9 +558,9 @@ bool fused_variant_ok(int n, int v) {
 
 template <class T>
 void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
-                        const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
-                        const T* const* gval, Layer4<T> in, Layer4<T> tin, Layer4<T> out,
-                        Layer4<T> tout, int merges, int variant, hipStream_t st) {
+                        const uint8_t* tflags, const T* const* gval, Layer4<T> in, Layer4<T> tin,
+                        Layer4<T> out, Layer4<T> tout, int merges, int bsync, int variant,
+                        hipStream_t st) {
     if (w.nblocks == 0) return;
     const unsigned blocks = (unsigned)w.nblocks;
     if (axis == AX_Z) {
@@ -563,8 +577,8 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
         const size_t lds = park ? (size_t)(nthreads / 64) * 64 * 4 * ((m + vw - 1) / vw) * 16 : 0;
 #define LZ(MM, W, PK, LL)                                                                          \
     hipLaunchKernelGGL((fused_z_kernel<T, MM, W, PK, LL>), dim3(blocks), dim3(nthreads), lds, st, \
-                       insts, three_dt, type, bcv, bct, gval[0], gval[1], gval[2], gval[3], in,     \
-                       tin, out, tout, merges, w.d_items, w.d_inst, w.vec_ok)
+                       insts, three_dt, tflags, gval[0], gval[1], gval[2], gval[3], in, tin, out,   \
+                       tout, merges, bsync, w.d_items, w.d_inst, w.vec_ok)
 #define LZM(W, PK)                      \
     if (!wide) {                        \
         if (m == 2) LZ(2, W, PK, 32);   \
@@ -598,8 +612,8 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
     hipLaunchKernelGGL((fused_strided_kernel<T, AX, kShapes[V].chunk, kShapes[V].bytes / (int)sizeof(T), \
                                              32 * (kShapes[V].bytes / (int)sizeof(T)),            \
                                              16 / (int)sizeof(T), kShapes[V].keep>),              \
-                       dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, type, bcv, bct,  \
-                       gval[0], gval[1], gval[2], gval[3], in, tin, out, tout, merges, w.d_items)
+                       dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, tflags, gval[0], \
+                       gval[1], gval[2], gval[3], in, tin, out, tout, merges, bsync, w.d_items)
 #define LSA(AX)                        \
     switch (variant) {                 \
         case 0: LS(AX, 0); break;      \
@@ -710,6 +724,14 @@ int shared_free_axes(const uint8_t* t, const uint8_t* bv, const uint8_t* bt, con
     return mask;
 }
 
+void build_node_flags(const uint8_t* t, const uint8_t* bv, const uint8_t* bt, int64_t n,
+                      uint8_t* out) {
+    for (int64_t i = 0; i < n; i++)
+        out[i] = t[i] == NODE_IN ? (uint8_t)0
+                                 : (uint8_t)(t[i] | (bv[i] == BC_FREE ? kFlagFreeVel : 0) |
+                                             (bt[i] == BC_FREE ? kFlagFreeT : 0));
+}
+
 template bool fused_supported<float>(const Geom&);
 template bool fused_supported<double>(const Geom&);
 template bool fused_variant_ok<float>(int, int);
@@ -717,12 +739,12 @@ template bool fused_variant_ok<double>(int, int);
 template int strided_lpg<float>(int);
 template int strided_lpg<double>(int);
 template void launch_fused_sweep<float>(const InstDesc<float>*, float, int, const FusedWork&,
-                                        const uint8_t*, const uint8_t*, const uint8_t*,
-                                        const float* const*, Layer4<float>, Layer4<float>,
-                                        Layer4<float>, Layer4<float>, int, int, hipStream_t);
+                                        const uint8_t*, const float* const*, Layer4<float>,
+                                        Layer4<float>, Layer4<float>, Layer4<float>, int, int, int,
+                                        hipStream_t);
 template void launch_fused_sweep<double>(const InstDesc<double>*, double, int, const FusedWork&,
-                                         const uint8_t*, const uint8_t*, const uint8_t*,
-                                         const double* const*, Layer4<double>, Layer4<double>,
-                                         Layer4<double>, Layer4<double>, int, int, hipStream_t);
+                                         const uint8_t*, const double* const*, Layer4<double>,
+                                         Layer4<double>, Layer4<double>, Layer4<double>, int, int,
+                                         int, hipStream_t);
 
 }  // namespace cmcx

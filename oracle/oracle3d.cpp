@@ -21,6 +21,7 @@
 //                         src/FluidSolver3D/TimeLayer3D.h:926-951
 //   div_error             src/FluidSolver3D/TimeLayer3D.h:595-641
 //   get_layer             src/FluidSolver3D/Solver3D.cpp:21-25, TimeLayer3D.h:819-924
+//   set_boundary          src/FluidSolver3D/Grid3D.h:121 (Grid3D::SetData)
 //
 // Merge modes:
 //   0 = CPU semantics (AdiSolver3D.cpp:591-604, 634-655, 347-357): the 4 fields
@@ -816,6 +817,22 @@ void orc_set_fields(void* h, const double* u, const double* v, const double* w, 
         for (int c = 0; c < 4; c++)
             for (size_t i = 0; i < s->N; i++) s->cur->f[c][s->cur->halo + i] = (decltype(s->dx))in[c][i];
         s->copy_state_to_all();
+    });
+}
+
+// Grid3D::SetData (Grid3D.h:121): new BC kinds and node values, read by the
+// boundary rows (and UpdateBoundaries) from the next step on.
+void orc_set_boundary(void* h, const uint8_t* bv, const uint8_t* bt, const double* vx,
+                      const double* vy, const double* vz, const double* T) {
+    DISPATCH(h, {
+        s->bc_vel.assign(bv, bv + s->N);
+        s->bc_temp.assign(bt, bt + s->N);
+        for (size_t i = 0; i < s->N; i++) {
+            s->gv[0][i] = (decltype(s->dx))vx[i];
+            s->gv[1][i] = (decltype(s->dx))vy[i];
+            s->gv[2][i] = (decltype(s->dx))vz[i];
+            s->gT[i] = (decltype(s->dx))T[i];
+        }
     });
 }
 

@@ -52,6 +52,8 @@ def load():
     L.orc_get_layer_raw.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_d, _c_d, _c_d, _c_d]
     L.orc_get_layer_raw.restype = None
     L.orc_set_fields.argtypes = [ctypes.c_void_p, _c_d, _c_d, _c_d, _c_d]
+    L.orc_set_boundary.argtypes = [ctypes.c_void_p, _c_u8, _c_u8, _c_d, _c_d, _c_d, _c_d]
+    L.orc_set_boundary.restype = None
     L.orc_set_fields.restype = None
     L.orc_div_error_cur.argtypes = [ctypes.c_void_p]
     L.orc_div_error_cur.restype = ctypes.c_double
@@ -171,6 +173,13 @@ class Oracle3D:
     def set_fields(self, u, v, w, T):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, T)]
         self.L.orc_set_fields(self.h, *[_ptr(a, _c_d) for a in arrs])
+
+    def set_boundary(self, bc_vel, bc_temp, vx, vy, vz, T):
+        bv = np.ascontiguousarray(bc_vel, dtype=np.uint8)
+        bt = np.ascontiguousarray(bc_temp, dtype=np.uint8)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, vz, T)]
+        self.L.orc_set_boundary(self.h, _ptr(bv, _c_u8), _ptr(bt, _c_u8),
+                                *[_ptr(a, _c_d) for a in arrs])
 
     def div_error(self):
         return self.L.orc_div_error_cur(self.h)

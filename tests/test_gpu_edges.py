@@ -123,3 +123,60 @@ def test_longer_lines_rejected(dtype, shape):
     s.UpdateBoundaries()
     s.TimeStep(d.dt, 1, 1, False)
     s.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_boundary_values_out_of_sync(dtype, graph):
+    """Boundary rows take the node values (Grid3D Node data), whatever the state
+    holds on BOUND/VALVE nodes: after set_fields and after set_boundary without
+    UpdateBoundaries the state differs from the node values there, and the
+    fused sweeps read the node-value arrays; after UpdateBoundaries they read
+    the same values from the Picard layer.  Both against the oracle."""
+    rng = np.random.default_rng(21)
+    d = domains.random_domain((14, 12, 16), rng, 0.12, free_faces=True)
+    n = d.n
+    say(f"[bsync] {dtype} graph={graph}")
+    ref = oracle_lib.Oracle3D(d, "f64")
+    ref.create_segments()
+    s = make_solver(d, dtype, path=FUSED)
+    s.set_graph(graph)
+    tol = 1e-11 if dtype == "f64" else 1e-5
+
+    def check(tag):
+        got = s.fields()
+        want = ref.fields()
+        vs = vel_norm(*want[:3])
+        for k in range(3):
+            e = rel_l2_scaled(got[k], want[k], vs)
+            say(f"  {tag} field {k}: {e:.3e}")
+            assert e < tol, (tag, k)
+        assert rel_l2(got[4], want[3]) < tol, (tag, "T")
+
+    # 1. a state that differs from the node values everywhere, no boundary fill
+    st = [rng.normal(0.0, 0.3, n) for _ in range(3)] + [1.0 + 0.1 * rng.normal(size=n)]
+    s.set_fields(*st)
+    ref.set_fields(*st)
+    for _ in range(2):
+        s.TimeStep(d.dt, 2, 1, False)
+        r, _ = ref.step(d.dt, 2, 1, False)
+        assert r == 0
+    check("set_fields, no fill")
+    # 2. new node values, no boundary fill
+    bv = [d.vx + 0.2 * rng.normal(size=n), d.vy + 0.2 * rng.normal(size=n),
+          d.vz + 0.2 * rng.normal(size=n), d.T + 0.1 * rng.normal(size=n)]
+    s.set_boundary(d.bc_vel, d.bc_temp, *bv)
+    ref.set_boundary(d.bc_vel, d.bc_temp, *bv)
+    s.TimeStep(d.dt, 2, 1, False)
+    r, _ = ref.step(d.dt, 2, 1, False)
+    assert r == 0
+    check("set_boundary, no fill")
+    # 3. boundary fill, then steps in sync
+    for _ in range(2):
+        s.UpdateBoundaries()
+        ref.update_boundaries()
+        s.TimeStep(d.dt, 2, 1, False)
+        r, _ = ref.step(d.dt, 2, 1, False)
+        assert r == 0
+    check("filled")
+    s.close()
