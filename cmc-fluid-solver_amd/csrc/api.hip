@@ -123,6 +123,18 @@ class Solver final : public ISolver {
     uint32_t* d_nonin = nullptr;
     uint32_t* d_bvlist = nullptr;
     int64_t n_nonin = 0, n_bv = 0;
+    // BOUND/VALVE nodes with a free BC (velocity or T): while the state is in
+    // sync with the node values, the only non-IN nodes whose values move from
+    // step to step (capacity n_bv; rebuilt when the BC kinds change)
+    uint32_t* d_bvfree = nullptr;
+    int64_t n_bvfree = 0;
+    // next, tA and tB hold the state's values on every BOUND/VALVE node outside
+    // d_bvfree: the step set-up need only visit d_bvfree.  True at creation
+    // (every layer a copy of the state); a step in sync with the node values
+    // keeps it (its no-slip rows write the value they read from the Picard
+    // layer, itself the state's); a boundary fill out of sync, a step out of
+    // sync, new BC kinds or a state handed out by pointer clear it.
+    bool layers_ok = true;
     // HIP graphs of whole steps, per (boundary fill, dt, G, L, state buffer)
     bool graph_mode = false;
     // (with the boundary-sync mode the sweeps were captured with)
@@ -220,6 +232,7 @@ class Solver final : public ISolver {
             n_bv = (int64_t)bvl.size();
             CMCX_HIP(hipMalloc((void**)&d_nonin, sizeof(uint32_t) * std::max<int64_t>(n_nonin, 1)));
             CMCX_HIP(hipMalloc((void**)&d_bvlist, sizeof(uint32_t) * std::max<int64_t>(n_bv, 1)));
+            CMCX_HIP(hipMalloc((void**)&d_bvfree, sizeof(uint32_t) * std::max<int64_t>(n_bv, 1)));
             if (n_nonin)
                 CMCX_HIP(hipMemcpyAsync(d_nonin, nonin.data(), sizeof(uint32_t) * n_nonin,
                                         hipMemcpyHostToDevice, st));
@@ -232,7 +245,24 @@ class Solver final : public ISolver {
             copy_state_to_all(0, NT);
             CMCX_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
         }
+        upload_free_list();
         upload_desc();
+    }
+
+    void upload_free_list() {
+        std::vector<uint32_t> fl;
+        for (const Grid& G : grids)
+            for (int64_t k = 0; k < G.g.N; k++) {
+                const int64_t i = G.off + k;
+                const uint8_t t = h_type[i];
+                if ((t == NODE_BOUND || t == NODE_VALVE) && (h_bcv[i] == BC_FREE || h_bct[i] == BC_FREE))
+                    fl.push_back((uint32_t)i);
+            }
+        n_bvfree = (int64_t)fl.size();
+        if (n_bvfree)
+            CMCX_HIP(hipMemcpyAsync(d_bvfree, fl.data(), sizeof(uint32_t) * n_bvfree,
+                                    hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));  // fl goes out of scope
     }
 
     void upload_flags(int64_t off, int64_t n) {
@@ -284,6 +314,7 @@ class Solver final : public ISolver {
         hipFree(d_part);
         hipFree(d_nonin);
         hipFree(d_bvlist);
+        hipFree(d_bvfree);
         hipFree(adj.ck);
         hipFree(adj.tape);
         hipFree(adj.work);
@@ -460,9 +491,24 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
-    int update_boundaries() override {
-        launch_update_boundaries_list<T>(n_bv, d_bvlist, d_g, cur, st);
+    // Boundary fill (AdiSolver3D.cpp:286-295): node values into the state on
+    // BOUND and VALVE nodes.  While the state is in sync, its no-slip nodes
+    // already hold them and only the free ones are visited.
+    void enqueue_update_boundaries(bool fast) {
+        if (fast) launch_update_boundaries_list<T>(n_bvfree, d_bvfree, d_g, cur, st);
+        else launch_update_boundaries_list<T>(n_bv, d_bvlist, d_g, cur, st);
+    }
+
+    // The flags after a boundary fill; returns whether the fill may be the short one.
+    bool note_update_boundaries() {
+        const bool fast = bv_synced;
+        if (!fast) layers_ok = false;  // no-slip state values change under next, tA, tB
         bv_synced = true;
+        return fast;
+    }
+
+    int update_boundaries() override {
+        enqueue_update_boundaries(note_update_boundaries());
         return CMCX_OK;
     }
 
@@ -535,10 +581,14 @@ class Solver final : public ISolver {
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
 
-    // Enqueues the sweeps of one time step (no synchronisation, no host reads).
-    void enqueue_step(T dt, int G, int L) {
+    // Enqueues the sweeps of one time step (no synchronisation, no host reads);
+    // `short_prep`: the step set-up visits only the free BOUND/VALVE nodes.
+    void enqueue_step(T dt, int G, int L, bool short_prep) {
         if (merge_order == CMCX_MERGE_CPU) {
-            launch_prep_step_list<T>(n_nonin, d_nonin, d_type, cur, next, tA, tB, st);
+            // TimeStep set-up (AdiSolver3D.cpp:308-316): next <- cur on BOUND and
+            // VALVE nodes, temp <- cur (the sweeps write NODE_IN nodes of temp)
+            if (short_prep) launch_prep_step_list<T>(n_bvfree, d_bvfree, d_type, cur, next, tA, tB, st);
+            else launch_prep_step_list<T>(n_nonin, d_nonin, d_type, cur, next, tA, tB, st);
             Layer4<T> tin = cur;  // temp == cur at the first sweep
             auto other = [&](const Layer4<T>& t) { return t.f[0] == tB.f[0] ? tA : tB; };
             for (int it = 0; it < G; it++) {
@@ -575,10 +625,11 @@ class Solver final : public ISolver {
 
     // The graph of (boundary fill +) one step for the current state buffer;
     // captured on a private stream the first time, replayed on the solver's stream.
-    hipGraphExec_t step_graph(bool ub, T dt, int G, int L) {
-        if (ub) bv_synced = true;  // the graph's boundary fill precedes its sweeps
-        auto key = std::make_tuple(ub ? 1 : 0, (double)dt, G, L, (const void*)cur.f[0],
-                                   bv_synced ? 1 : 0);
+    // ub: 0 no fill, 1 the whole list, 2 the free nodes only; the sweeps read the
+    // boundary-sync mode from bv_synced, as set for after the fill.
+    hipGraphExec_t step_graph(int ub, bool short_prep, T dt, int G, int L) {
+        auto key = std::make_tuple(ub, (double)dt, G, L, (const void*)cur.f[0],
+                                   (short_prep ? 2 : 0) + (bv_synced ? 1 : 0));
         auto it = graphs.find(key);
         if (it != graphs.end()) return it->second;
         if (!cap) CMCX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -587,8 +638,8 @@ class Solver final : public ISolver {
         st = cap;
         CMCX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
         try {
-            if (ub) update_boundaries();
-            enqueue_step(dt, G, L);
+            if (ub) enqueue_update_boundaries(ub == 2);
+            enqueue_step(dt, G, L, short_prep);
         } catch (...) {
             hipStreamEndCapture(cap, &gr);
             if (gr) hipGraphDestroy(gr);
@@ -607,12 +658,17 @@ class Solver final : public ISolver {
 
     // (Boundary fill +) one step, replayed as a graph or launched directly.
     void launch_step(bool ub, T dt, int G, int L) {
+        const int ubm = ub ? (note_update_boundaries() ? 2 : 1) : 0;
+        const bool short_prep = layers_ok;
         if (graph_mode && !prof) {
-            CMCX_HIP(hipGraphLaunch(step_graph(ub, dt, G, L), st));
+            CMCX_HIP(hipGraphLaunch(step_graph(ubm, short_prep, dt, G, L), st));
         } else {
-            if (ub) update_boundaries();
-            enqueue_step(dt, G, L);
+            if (ubm) enqueue_update_boundaries(ubm == 2);
+            enqueue_step(dt, G, L, short_prep);
         }
+        // a step in sync writes the state's own values on no-slip nodes (the
+        // CUDA merge order's set-up leaves tB alone)
+        layers_ok = merge_order == CMCX_MERGE_CPU && bv_synced;
     }
 
     // Launches the step (graph or direct), then the divergence guard and swap.
@@ -721,6 +777,7 @@ class Solver final : public ISolver {
     int state_ptrs(void** p4) override {
         if (!p4) return CMCX_ERR_ARG;
         bv_synced = false;  // the caller may write the state
+        layers_ok = false;
         for (int f = 0; f < 4; f++) p4[f] = cur.f[f] + G().off;
         return CMCX_OK;
     }
@@ -780,7 +837,10 @@ class Solver final : public ISolver {
             CMCX_HIP(hipStreamSynchronize(st));  // tmp is reused
         }
         upload_flags(o, N);
+        upload_free_list();
         bv_synced = false;
+        layers_ok = false;
+        clear_graphs();  // the free-node count is captured
         if (built) {
             Gr.free_shared = shared_free_axes(&h_type[o], &h_bcv[o], &h_bct[o], Gr.g);
             prepare_path_buffers();
@@ -845,6 +905,7 @@ class Solver final : public ISolver {
             CMCX_HIP(hipEventDestroy(e1));
             tune_cache()[tune_key()] = {variant[0], variant[1], variant[2]};
             clear_graphs();
+            layers_ok = false;  // the timed sweeps wrote half and tA
         }
         if (choice)
             for (int a = 0; a < 3; a++) choice[a] = variant[a];
@@ -958,6 +1019,7 @@ class Solver final : public ISolver {
         }
         adj.recorded = true;
         adj.last_taped = true;
+        layers_ok = false;  // record_step leaves next and the Picard layers behind
         return CMCX_OK;
     }
 
@@ -1052,6 +1114,9 @@ class Solver final : public ISolver {
         }
         Layer4<T> XB = lay(adj.work, 0), GB = lay(adj.work, 4), FIN = lay(adj.work, 5);
         copy_layer(FIN, cur);
+        // the recomputed intervals start from checkpoints taken before their
+        // boundary fill: full fills there; the state and its flags come back at the end
+        const bool synced0 = bv_synced, layers0 = layers_ok;
         CMCX_HIP(hipMemsetAsync(adj.work, 0, sizeof(T) * 4 * NT * 5, st));
         CMCX_HIP(hipMemsetAsync(adj.pp, 0, sizeof(double) * 4 * adj_point_blocks(), st));
         if (seed)
@@ -1064,6 +1129,7 @@ class Solver final : public ISolver {
             const int k = std::min(adj.every, adj.steps - s0);
             if (sgm != nseg - 1 || !adj.last_taped) {
                 copy_layer(cur, lay(adj.ck, sgm));
+                bv_synced = false;
                 for (int i = 0; i < k; i++) record_step(adj.tape + i * per * 4 * NT);
             }
             for (int i = k - 1; i >= 0; i--) reverse_step(adj.tape + i * per * 4 * NT);
@@ -1079,6 +1145,8 @@ class Solver final : public ISolver {
         std::vector<double> pp(4 * adj_point_blocks());
         CMCX_HIP(hipMemcpyAsync(pp.data(), adj.pp, sizeof(double) * pp.size(), hipMemcpyDeviceToHost, st));
         copy_layer(cur, FIN);
+        bv_synced = synced0;
+        layers_ok = layers0;
         adj.last_taped = nseg == 1;  // the tape now holds the first interval
         CMCX_HIP(hipStreamSynchronize(st));
         if (pbar) {
