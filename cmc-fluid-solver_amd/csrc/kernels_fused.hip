@@ -201,6 +201,12 @@ __device__ __forceinline__ void write_phase(const Chunk<T, M>& ch, const int* ro
 // padding so that the lane groups of a wavefront (M rows apart) start on
 // different banks.
 __host__ __device__ constexpr int keep_pitch(int lpg, int m) { return lpg + (lpg >= m ? lpg / m : 0); }
+// 128-byte tiles leave no LDS for the padding (their parked layer and reduced
+// systems fill the 160 KB exactly)
+template <class T>
+__host__ __device__ constexpr int keep_row_pitch(int lpg, int m) {
+    return lpg * (int)sizeof(T) >= 128 ? lpg : keep_pitch(lpg, m);
+}
 
 // ---------------------------------------------------------------------------
 // Strided lines (X or Y sweep).  A work-group owns LPG consecutive z lines and
@@ -249,31 +255,65 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
         insts[inst].off + (AXIS == AX_X ? (int64_t)outer * g.sy : (int64_t)outer * g.sx) + z;
     const int r0 = chunk * M;
 
-    // KEEP: each lane parks the Picard layer of its rows in LDS after the
-    // reduced systems, [field][row][line] with a padded line pitch (the lane
-    // groups of a wave then hit different banks); only the lane itself reads it
-    const int krs = keep_pitch(LPG, M);
+    // KEEP: each lane parks the Picard layer of its rows in LDS, after the
+    // reduced systems, [field][row][line], with a padded line pitch (the lane
+    // groups of a wave then hit different banks) where the LDS has room for
+    // it; only the lane itself reads it.  The reduced systems then take the
+    // 8-value normalised form (kNormFields), which leaves 128-byte tiles room
+    // for the parked layer.
+    const int red_elems = KEEP ? LPG * kNormFields * pow2_at_least(P) : LPG * L.pitch;
+    const int krs = keep_row_pitch<T>(LPG, M);
     const int kfs = P * M * krs;
-    T* keep = red + LPG * L.pitch + r0 * krs + gl;
+    T* keep = red + red_elems + r0 * krs + gl;
     int role[M];
     Chunk<T, M> ch;
     T f[kRedFields], l[kRedFields];
     chunk_phase<T, AXIS, M, KEEP>(g, p, type, bsrc, in, tin, base, s, r0, n, zok, role, ch, f, l,
                                   keep, kfs, krs);
-#pragma unroll
-    for (int k = 0; k < kRedFields; k++) {
-        red[L.at(gl, k, 2 * chunk)] = f[k];
-        red[L.at(gl, k, 2 * chunk + 1)] = l[k];
-    }
-    __syncthreads();
-    solve_reduced_compact<T>(red, L, LPG, wave, nw);
-    __syncthreads();
-    if (!zok) return;
     T xf[4], xl[4];
+    if constexpr (KEEP) {
+        const NormLayout NL{pow2_at_least(P)};
+        T fh[kNormFields];
+        normalise_first<T>(f, fh);
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        xf[v] = red[L.at(gl, 6 + v, 2 * chunk)];
-        xl[v] = red[L.at(gl, 6 + v, 2 * chunk + 1)];
+        for (int k = 0; k < kNormFields; k++) red[NL.at(gl, k, chunk)] = fh[k];
+        __syncthreads();
+        {
+            T nx[kNormFields], cr[kNormFields];
+            const bool hn = chunk + 1 < P;
+#pragma unroll
+            for (int k = 0; k < kNormFields; k++) nx[k] = hn ? red[NL.at(gl, k, chunk + 1)] : (T)0;
+            compact_norm<T>(fh, l, nx, cr);
+            __syncthreads();  // every lane has read the first rows
+#pragma unroll
+            for (int k = 0; k < kNormFields; k++) red[NL.at(gl, k, chunk)] = cr[k];
+        }
+        __syncthreads();
+        solve_norm_lines<T>(red, NL, LPG, P, wave, nw);
+        __syncthreads();
+        if (!zok) return;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int q = v < 3 ? 0 : 1;
+            xl[v] = red[NL.at(gl, 4 + v, chunk)];
+            const T xlp = chunk > 0 ? red[NL.at(gl, 4 + v, chunk - 1)] : (T)0;
+            xf[v] = fh[4 + v] - fh[2 * q] * xlp - fh[2 * q + 1] * xl[v];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRedFields; k++) {
+            red[L.at(gl, k, 2 * chunk)] = f[k];
+            red[L.at(gl, k, 2 * chunk + 1)] = l[k];
+        }
+        __syncthreads();
+        solve_reduced_compact<T>(red, L, LPG, wave, nw);
+        __syncthreads();
+        if (!zok) return;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            xf[v] = red[L.at(gl, 6 + v, 2 * chunk)];
+            xl[v] = red[L.at(gl, 6 + v, 2 * chunk + 1)];
+        }
     }
     write_phase<T, M, KEEP>(ch, role, base, s, r0, xf, xl, tin, out, tout, merges, keep, kfs, krs);
 }
@@ -487,15 +527,16 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
 // elimination and reduced solve), rows per chunk (8, or 4 for lines of at most
 // 128 nodes: twice the lanes per line; the reduced system keeps at most 64
 // rows), and whether the Picard layer read for the coefficients is parked in
-// LDS for the merge instead of re-read from memory.  Registers are capped at 4
-// (FP32) / 2 (FP64) waves per SIMD.
+// LDS for the merge instead of re-read from memory (variants 6-9: 0, 1, 2 and
+// 3 with the parked layer).  Registers are capped at 4 (FP32) / 2 (FP64)
+// waves per SIMD.
 struct StridedShape {
     int bytes, chunk;
     bool keep;
 };
 constexpr StridedShape kShapes[kFusedVariants] = {
     {128, 8, false}, {64, 8, false}, {32, 8, false}, {128, 4, false}, {64, 4, false},
-    {32, 4, false},  {64, 8, true},  {32, 8, true},  {64, 4, true},   {32, 4, true}};
+    {32, 4, false},  {128, 8, true}, {64, 8, true},  {32, 8, true},   {128, 4, true}};
 constexpr int kChunk = 8;
 template <class T>
 constexpr int variant_chunk(int variant) { return kShapes[variant].chunk; }
@@ -528,9 +569,9 @@ template <class T>
 size_t strided_lds(int n, int v) {
     const int lpg = variant_lpg<T>(v), m = variant_chunk<T>(v);
     const int P = strided_waves_for(n, lpg, m) * (64 / lpg);
-    size_t b = (size_t)lpg * (kRedFields * 2 * P + 1) * sizeof(T);
-    if (kShapes[v].keep) b += (size_t)4 * P * m * keep_pitch(lpg, m) * sizeof(T);
-    return b;
+    if (!kShapes[v].keep) return (size_t)lpg * (kRedFields * 2 * P + 1) * sizeof(T);
+    return (size_t)lpg * kNormFields * pow2_at_least(P) * sizeof(T) +
+           (size_t)4 * P * m * keep_row_pitch<T>(lpg, m) * sizeof(T);
 }
 
 // A strided variant fits lines of n nodes when its work-group (one lane per

@@ -329,6 +329,87 @@ __device__ __forceinline__ void solve_reduced_compact(T* red, RedLayout L, int n
     }
 }
 
+// Reduced system in 8 values per chunk, for kernels whose LDS is mostly taken
+// by other data: the chunk's lane divides its first row by b and stores it;
+// after a barrier it reads the next chunk's normalised first row, eliminates
+// both first unknowns from its last row (the compaction step above) and stores
+// that row, again divided by its b.  PCR then runs over one row per chunk, and
+// each lane recovers x_f from its own normalised first row.  Row i of line
+// `line` is at NormLayout::at; chunks are XOR-swizzled by the line so that the
+// chunk-major stores (lanes = lines) and the line-major PCR loads (lanes =
+// chunks) both spread over the banks.
+constexpr int kNormFields = 8;  // a_v c_v a_t c_t d_u d_v d_w d_t, over the row's b
+struct NormLayout {
+    int P2;  // chunks per line rounded up to a power of two
+    __device__ __forceinline__ int at(int line, int field, int chunk) const {
+        return (line * kNormFields + field) * P2 + (chunk ^ (line & (P2 - 1)));
+    }
+};
+
+__host__ __device__ constexpr int pow2_at_least(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// First row (kRedFields layout) -> normalised: x_f = d - a x_l(prev) - c x_l.
+template <class T>
+__device__ __forceinline__ void normalise_first(const T* f, T* fh) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const T ib = rcp(f[3 * q + 1]);
+        fh[2 * q] = f[3 * q] * ib;
+        fh[2 * q + 1] = f[3 * q + 2] * ib;
+#pragma unroll
+        for (int v = (q == 0 ? 0 : 3); v < (q == 0 ? 3 : 4); v++) fh[4 + v] = f[6 + v] * ib;
+    }
+}
+
+// Last row l with this chunk's (fh) and the next chunk's (nx, zeros past the
+// line's end) normalised first rows substituted; normalised result in cr.
+template <class T>
+__device__ __forceinline__ void compact_norm(const T* fh, const T* l, const T* nx, T* cr) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const T al = l[3 * q], bl = l[3 * q + 1], cl = l[3 * q + 2];
+        const T iB = rcp(bl - al * fh[2 * q + 1] - cl * nx[2 * q]);
+        cr[2 * q] = -al * fh[2 * q] * iB;
+        cr[2 * q + 1] = -cl * nx[2 * q + 1] * iB;
+#pragma unroll
+        for (int v = (q == 0 ? 0 : 3); v < (q == 0 ? 3 : 4); v++)
+            cr[4 + v] = (l[6 + v] - al * fh[4 + v] - cl * nx[4 + v]) * iB;
+    }
+}
+
+// PCR over the compacted rows of `nl` lines (P chunks each) with the lanes of
+// `nw` wavefronts (this one is wave `w`), the lanes of a line inside one
+// wavefront; x_l replaces the right-hand sides (fields 4..7) in place.
+template <class T>
+__device__ __forceinline__ void solve_norm_lines(T* red, NormLayout NL, int nl, int P, int w, int nw) {
+    const int lane = threadIdx.x & 63;
+    const int RP = NL.P2;
+    const int lpp = 64 / RP;
+    const int i = lane % RP;
+    for (int base = w * lpp; base < nl; base += nw * lpp) {
+        const int line = base + lane / RP;
+        const bool act = line < nl && i < P;
+        T A[2], B[2], C[2], D[4];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            A[q] = act ? red[NL.at(line, 2 * q, i)] : (T)0;
+            B[q] = (T)1;
+            C[q] = act ? red[NL.at(line, 2 * q + 1, i)] : (T)0;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) D[v] = act ? red[NL.at(line, 4 + v, i)] : (T)0;
+        pcr_solve<T>(A, B, C, D, i, line < nl ? P : 0, RP);
+        if (act) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[NL.at(line, 4 + v, i)] = D[v];
+        }
+    }
+}
+
 // 16-byte vector of consecutive elements.
 template <class T>
 struct Vec16;

@@ -198,7 +198,7 @@ int cmcx_autotune(cmcx_solver* s, int reps, int* choice);
    0 = 128-byte z rows per work-group (one work-group per CU), 1 = 64-byte rows
    (two work-groups per CU), 2 = 32-byte rows (four per CU); 3, 4, 5 = the same
    with half the rows per lane (lines of at most 128 nodes); 6, 7, 8, 9 =
-   variants 1, 2, 4, 5 keeping the Picard layer in LDS between the elimination
+   variants 0, 1, 2, 3 keeping the Picard layer in LDS between the elimination
    and the merge (variants that do not fit the grid return
    CMCX_ERR_UNSUPPORTED).  Z sweep (2), 0-7: bit 0 = registers capped for more
    waves per SIMD, bit 1 = the Picard layer parked in LDS until the merge,

@@ -104,11 +104,13 @@ def test_longest_z_lines(dtype, nz):
 
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_longest_strided_lines(dtype):
-    """X and Y lines of 256 nodes: the 128-, 64- and 32-byte tile variants."""
+    """X and Y lines of 256 nodes: the 128-, 64- and 32-byte tile variants, with
+    the Picard layer re-read (0-2) and parked in LDS (6-8; the 128-byte one
+    fills the 160 KB of LDS exactly)."""
     for shape in ((256, 9, 12), (9, 256, 12)):
         d = _long(shape, 5)
         say(f"[long strided] {dtype} {shape}")
-        _compare(d, dtype, FUSED, [(v, v, 0) for v in (0, 1, 2)])
+        _compare(d, dtype, FUSED, [(v, v, 0) for v in (0, 1, 2, 6, 7, 8)])
 
 
 @pytest.mark.parametrize("dtype,shape", [("f32", (8, 8, 1024)), ("f64", (8, 8, 512)),
