@@ -116,11 +116,13 @@ __global__ void __launch_bounds__(256) div_error_kernel(Geom g, const uint8_t* _
                                                         double* __restrict__ part) {
     double val = 0.0, cnt = 0.0;
     const int64_t sx = g.sx, sy = g.sy;
-    for (int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; id < g.N;
-         id += (int64_t)gridDim.x * blockDim.x) {
-        int k = (int)(id % g.nz);
-        int j = (int)((id / g.nz) % g.ny);
-        int i = (int)(id / g.sx);
+    // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit index math
+    const uint32_t N = (uint32_t)g.N, nz = (uint32_t)g.nz, ny = (uint32_t)g.ny;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < N; id += gridDim.x * blockDim.x) {
+        const uint32_t line = id / nz;
+        const int k = (int)(id - line * nz);
+        const int i = (int)(line / ny);
+        const int j = (int)(line - (uint32_t)i * ny);
         if (i < g.nx - 1 && j < g.ny - 1 && k < g.nz - 1 && type[id] == NODE_IN) {
             const int64_t c = id;
             T su = U[c] + U[c - sy] + U[c - sy - 1] + U[c - 1] - U[c - sx] - U[c - sx - sy] -
@@ -153,15 +155,24 @@ __global__ void __launch_bounds__(256) div_error_kernel(Geom g, const uint8_t* _
     }
 }
 
-__global__ void div_finish_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wavefront: lane t sums the partials t, t + 64, ... in order, then a
+// fixed butterfly over the lanes (the same order on every run).
+__global__ void __launch_bounds__(64) div_finish_kernel(const double* __restrict__ part, int nb,
+                                                        double* __restrict__ out) {
+    const int t = threadIdx.x;
     double s = 0.0, c = 0.0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = t; b < nb; b += 64) {
         s += part[2 * b];
         c += part[2 * b + 1];
     }
-    out[0] = s;
-    out[1] = c;
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        c += __shfl_xor(c, o, 64);
+    }
+    if (t == 0) {
+        out[0] = s;
+        out[1] = c;
+    }
 }
 
 template <class T>
