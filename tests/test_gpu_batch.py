@@ -51,9 +51,12 @@ def _oracle_fields(d, dtype, steps, G):
     return f, errs
 
 
-def _check_batch(doms, dtype, path, steps=3, G=2):
+def _check_batch(doms, dtype, path, steps=3, G=2, variant=None):
     s = make_batch(doms, dtype, path=path)
     assert len(s) == len(doms)
+    if variant is not None:
+        for a in (0, 1):
+            s.set_tile_variant(a, variant)
     dt = doms[0].dt
     last = None
     for i in range(steps):
@@ -83,6 +86,17 @@ def test_batch_matches_oracle(dtype, path):
     doms = _boxes()
     say(f"[batch] {len(doms)} grids {[d.shape for d in doms]} {dtype} path={path}")
     _check_batch(doms, dtype, path)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("variant", [6, 7, 9])
+def test_batch_lds_parked_variants(variant, dtype):
+    """X and Y sweeps with the Picard layer parked in LDS on grids of mixed
+    sizes in one launch (the work-group is sized for the longest line; the
+    shorter lines end in identity chunks)."""
+    doms = _boxes()
+    say(f"[batch variant {variant}] {dtype}")
+    _check_batch(doms, dtype, FUSED, variant=variant)
 
 
 def test_batch_with_shared_free_node():
