@@ -734,6 +734,16 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
                 }
         }
     }
+    if (axis == AX_Y) {
+        // The Y sweep reads what the Z sweep wrote: x-planes, in list order
+        // per XCD.  Walking its own list backwards, it starts on the planes Z
+        // wrote last, which the memory-side cache still holds.
+        const size_t np = items.size() / 2;
+        for (size_t k = 0; k < np / 2; k++) {
+            std::swap(items[2 * k], items[2 * (np - 1 - k)]);
+            std::swap(items[2 * k + 1], items[2 * (np - 1 - k) + 1]);
+        }
+    }
     w.nblocks = axis == AX_Z ? (int64_t)winst.size() : (int64_t)items.size() / 2;
     if (!items.empty()) {
         CMCX_HIP(hipMalloc((void**)&w.d_items, sizeof(int32_t) * items.size()));

@@ -171,7 +171,10 @@ int cmcx_get_fields(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T)
 int cmcx_get_fields_device(cmcx_solver* s, void* u, void* v, void* w, void* p, void* T);
 /* Replace the current state (initial conditions, checkpoint restore). */
 int cmcx_set_fields(cmcx_solver* s, const void* u, const void* v, const void* w, const void* T);
-/* Device pointers of the current state fields {u, v, w, T} (valid until the next step). */
+/* Device pointers of the current state fields {u, v, w, T} (valid until the next step).
+   The solver takes the state as rewritten at this call (the next boundary fill and
+   step set-up then visit every BOUND/VALVE and non-IN node): write through the
+   pointers before the next cmcx_update_boundaries, cmcx_step or cmcx_run. */
 int cmcx_state_device_ptrs(cmcx_solver* s, void** ptrs4);
 
 /* Solver3D::GetLayer: the reference output view.  Sets NODE_OUT nodes of the
