@@ -327,11 +327,12 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
     const T* W = tin.f[2];
     const T* Tt = tin.f[3];
     double part[4] = {0, 0, 0, 0};
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.N;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(q / sx);
-        const int y = (int)((q - x * sx) / sy);
-        const int z = (int)(q - x * sx - y * sy);
+    // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit index math
+    const uint32_t N = (uint32_t)g.N, usx = (uint32_t)sx, usy = (uint32_t)sy;
+    for (uint32_t uq = blockIdx.x * blockDim.x + threadIdx.x; uq < N; uq += gridDim.x * blockDim.x) {
+        const int64_t q = uq;
+        const uint32_t ux = uq / usx, rem = uq - ux * usx, uy = rem / usy;
+        const int x = (int)ux, y = (int)uy, z = (int)(rem - uy * usy);
         const uint8_t tq = type[q];
         const bool inq = tq == NODE_IN;
         T acc[4], acc_state[4] = {0, 0, 0, 0};
