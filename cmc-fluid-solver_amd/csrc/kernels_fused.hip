@@ -364,8 +364,8 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
 #pragma unroll
     for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? type[c0 + j] : kOutside;
     {
-        int prev = __shfl((int)ty[M], lane - 1, 64);
-        int next = __shfl((int)ty[1], lane + 1, 64);
+        int prev = lane_prev((int)ty[M]);
+        int next = lane_next((int)ty[1]);
         ty[0] = (ll > 0 && ok) ? (uint8_t)prev : kOutside;
         ty[M + 1] = (ll < LPL - 1 && ok && r0 + M < n) ? (uint8_t)next : kOutside;
     }
@@ -415,8 +415,8 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     T tprev[4], tnext[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        tprev[v] = __shfl(tv[v][M - 1], lane - 1, 64);
-        tnext[v] = __shfl(tv[v][0], lane + 1, 64);
+        tprev[v] = lane_prev(tv[v][M - 1]);
+        tnext[v] = lane_next(tv[v][0]);
     }
     using V16 = typename Vec16<T>::type;
     constexpr int VW = Vec16<T>::n;

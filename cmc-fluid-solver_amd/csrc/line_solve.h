@@ -142,6 +142,24 @@ __device__ __forceinline__ void chunk_reduced(const Chunk<T, M>& ch, const Row<T
     }
 }
 
+// Values of the neighbouring lanes of the wavefront (lane - 1, lane + 1) by
+// DPP wave shifts, which move data in the VALU instead of through the LDS
+// crossbar like __shfl; lane 0 (lane 63) receives 0.
+__device__ __forceinline__ int lane_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev(__float_as_int(x))); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(lane_next(__float_as_int(x))); }
+__device__ __forceinline__ double lane_prev(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)lane_prev((int)(unsigned)b), hi = (unsigned)lane_prev((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_next(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)lane_next((int)(unsigned)b), hi = (unsigned)lane_next((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Parallel cyclic reduction over one reduced system per group of RP lanes:
 // the lane holds row i = lane % RP (rows >= R are identity, and every access
 // stays inside the lane's group).  On return D holds x.
@@ -160,13 +178,23 @@ __device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, 
         const bool hm = i - st >= 0, hp = i + st < RP;
         const int lm = hm ? lane - st : lane, lp = hp ? lane + st : lane;
         T Am[2], Bm[2], Cm[2], Ap[2], Bp[2], Cp[2], Dm[4], Dp[4];
+        if (st == 1) {
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            Am[q] = __shfl(A[q], lm, 64); Bm[q] = __shfl(B[q], lm, 64); Cm[q] = __shfl(C[q], lm, 64);
-            Ap[q] = __shfl(A[q], lp, 64); Bp[q] = __shfl(B[q], lp, 64); Cp[q] = __shfl(C[q], lp, 64);
+            for (int q = 0; q < 2; q++) {
+                Am[q] = lane_prev(A[q]); Bm[q] = lane_prev(B[q]); Cm[q] = lane_prev(C[q]);
+                Ap[q] = lane_next(A[q]); Bp[q] = lane_next(B[q]); Cp[q] = lane_next(C[q]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) { Dm[v] = lane_prev(D[v]); Dp[v] = lane_next(D[v]); }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                Am[q] = __shfl(A[q], lm, 64); Bm[q] = __shfl(B[q], lm, 64); Cm[q] = __shfl(C[q], lm, 64);
+                Ap[q] = __shfl(A[q], lp, 64); Bp[q] = __shfl(B[q], lp, 64); Cp[q] = __shfl(C[q], lp, 64);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) { Dm[v] = __shfl(D[v], lm, 64); Dp[v] = __shfl(D[v], lp, 64); }
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++) { Dm[v] = __shfl(D[v], lm, 64); Dp[v] = __shfl(D[v], lp, 64); }
         if (!hm) {
 #pragma unroll
             for (int q = 0; q < 2; q++) { Am[q] = 0; Bm[q] = 1; Cm[q] = 0; }
@@ -255,7 +283,7 @@ __device__ __forceinline__ void solve_chunks_in_lanes(const T* f, const T* l, in
     T fn[kRedFields];
 #pragma unroll
     for (int k = 0; k < kRedFields; k++) {
-        const T v = __shfl(f[k], hn ? lane + 1 : lane, 64);
+        const T v = lane_next(f[k]);
         fn[k] = hn ? v : ((k == 1 || k == 4) ? (T)1 : (T)0);
     }
     T A[2], B[2], C[2], D[4], ib[2];
@@ -264,7 +292,7 @@ __device__ __forceinline__ void solve_chunks_in_lanes(const T* f, const T* l, in
     T xlp[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const T t = __shfl(D[v], i > 0 ? lane - 1 : lane, 64);
+        const T t = lane_prev(D[v]);
         xlp[v] = i > 0 ? t : (T)0;
         xl[v] = D[v];
     }
@@ -312,7 +340,7 @@ __device__ __forceinline__ void solve_reduced_compact(T* red, RedLayout L, int n
         T xlp[4], xl[4], xf[4];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const T t = __shfl(D[v], i > 0 ? lane - 1 : lane, 64);
+            const T t = lane_prev(D[v]);
             xlp[v] = i > 0 ? t : (T)0;
             xl[v] = D[v];
         }
