@@ -165,7 +165,9 @@ __device__ __forceinline__ double lane_next(double x) {
 // stays inside the lane's group).  On return D holds x.
 template <class T>
 __device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, int RP) {
-    // A[q], B[q], C[q]: matrix q (0 vel, 1 T); D[0..2] use matrix 0, D[3] matrix 1
+    // A[q], B[q], C[q]: matrix q (0 vel, 1 T); D[0..2] use matrix 0, D[3] matrix 1.
+    // Rows are kept divided by their diagonal (B = 1), so a level exchanges A, C
+    // and D only and takes one reciprocal per matrix.
     if (i >= R) {
         A[0] = A[1] = C[0] = C[1] = 0;
         B[0] = B[1] = 1;
@@ -173,59 +175,60 @@ __device__ __forceinline__ void pcr_solve(T* A, T* B, T* C, T* D, int i, int R, 
     }
     if (i == 0) A[0] = A[1] = 0;
     if (i == R - 1) C[0] = C[1] = 0;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const T ib = rcp(B[q]);
+        A[q] = A[q] * ib;
+        C[q] = C[q] * ib;
+#pragma unroll
+        for (int v = (q == 0 ? 0 : 3); v < (q == 0 ? 3 : 4); v++) D[v] = D[v] * ib;
+    }
     const int lane = threadIdx.x & 63;
     for (int st = 1; st < RP; st <<= 1) {
         const bool hm = i - st >= 0, hp = i + st < RP;
         const int lm = hm ? lane - st : lane, lp = hp ? lane + st : lane;
-        T Am[2], Bm[2], Cm[2], Ap[2], Bp[2], Cp[2], Dm[4], Dp[4];
+        T Am[2], Cm[2], Ap[2], Cp[2], Dm[4], Dp[4];
         if (st == 1) {
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                Am[q] = lane_prev(A[q]); Bm[q] = lane_prev(B[q]); Cm[q] = lane_prev(C[q]);
-                Ap[q] = lane_next(A[q]); Bp[q] = lane_next(B[q]); Cp[q] = lane_next(C[q]);
+                Am[q] = lane_prev(A[q]); Cm[q] = lane_prev(C[q]);
+                Ap[q] = lane_next(A[q]); Cp[q] = lane_next(C[q]);
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) { Dm[v] = lane_prev(D[v]); Dp[v] = lane_next(D[v]); }
         } else {
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                Am[q] = __shfl(A[q], lm, 64); Bm[q] = __shfl(B[q], lm, 64); Cm[q] = __shfl(C[q], lm, 64);
-                Ap[q] = __shfl(A[q], lp, 64); Bp[q] = __shfl(B[q], lp, 64); Cp[q] = __shfl(C[q], lp, 64);
+                Am[q] = __shfl(A[q], lm, 64); Cm[q] = __shfl(C[q], lm, 64);
+                Ap[q] = __shfl(A[q], lp, 64); Cp[q] = __shfl(C[q], lp, 64);
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) { Dm[v] = __shfl(D[v], lm, 64); Dp[v] = __shfl(D[v], lp, 64); }
         }
         if (!hm) {
 #pragma unroll
-            for (int q = 0; q < 2; q++) { Am[q] = 0; Bm[q] = 1; Cm[q] = 0; }
+            for (int q = 0; q < 2; q++) { Am[q] = 0; Cm[q] = 0; }
 #pragma unroll
             for (int v = 0; v < 4; v++) Dm[v] = 0;
         }
         if (!hp) {
 #pragma unroll
-            for (int q = 0; q < 2; q++) { Ap[q] = 0; Bp[q] = 1; Cp[q] = 0; }
+            for (int q = 0; q < 2; q++) { Ap[q] = 0; Cp[q] = 0; }
 #pragma unroll
             for (int v = 0; v < 4; v++) Dp[v] = 0;
         }
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            T k1 = A[q] * rcp(Bm[q]);
-            T k2 = C[q] * rcp(Bp[q]);
-            B[q] = B[q] - Cm[q] * k1 - Ap[q] * k2;
-            if (q == 0) {
+            // row i - A row(i - st) - C row(i + st), divided again by its diagonal
+            const T ib = rcp((T)1 - Cm[q] * A[q] - Ap[q] * C[q]);
 #pragma unroll
-                for (int v = 0; v < 3; v++) D[v] = D[v] - Dm[v] * k1 - Dp[v] * k2;
-            } else {
-                D[3] = D[3] - Dm[3] * k1 - Dp[3] * k2;
-            }
-            A[q] = -Am[q] * k1;
-            C[q] = -Cp[q] * k2;
+            for (int v = (q == 0 ? 0 : 3); v < (q == 0 ? 3 : 4); v++)
+                D[v] = (D[v] - Dm[v] * A[q] - Dp[v] * C[q]) * ib;
+            const T a = -Am[q] * A[q], c = -Cp[q] * C[q];
+            A[q] = a * ib;
+            C[q] = c * ib;
         }
     }
-    const T i0 = rcp(B[0]);
-#pragma unroll
-    for (int v = 0; v < 3; v++) D[v] = D[v] * i0;
-    D[3] = D[3] * rcp(B[1]);
 }
 
 // Identity reduced row (x = 0) in the kRedFields layout.
