@@ -319,20 +319,19 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
                                                         Layer4<T> tb, Layer4<T> xb, Layer4<T> cacc,
                                                         Layer4<T> gb, int zmode, int last, T alpha,
                                                         double* __restrict__ pp) {
-    const int64_t sx = g.sx, sy = g.sy;
-    const int64_t s = AXIS == AX_X ? sx : AXIS == AX_Y ? sy : 1;
+    // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit indices
+    const uint32_t sx = (uint32_t)g.sx, sy = (uint32_t)g.sy;
+    const uint32_t s = AXIS == AX_X ? sx : AXIS == AX_Y ? sy : 1u;
     const T ix = p.inv_2hx, iy = p.inv_2hy, iz = p.inv_2hz;
     const T* U = tin.f[0];
     const T* V = tin.f[1];
     const T* W = tin.f[2];
     const T* Tt = tin.f[3];
     double part[4] = {0, 0, 0, 0};
-    // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit index math
-    const uint32_t N = (uint32_t)g.N, usx = (uint32_t)sx, usy = (uint32_t)sy;
-    for (uint32_t uq = blockIdx.x * blockDim.x + threadIdx.x; uq < N; uq += gridDim.x * blockDim.x) {
-        const int64_t q = uq;
-        const uint32_t ux = uq / usx, rem = uq - ux * usx, uy = rem / usy;
-        const int x = (int)ux, y = (int)uy, z = (int)(rem - uy * usy);
+    const uint32_t N = (uint32_t)g.N;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < N; q += gridDim.x * blockDim.x) {
+        const uint32_t ux = q / sx, rem = q - ux * sx, uy = rem / sy;
+        const int x = (int)ux, y = (int)uy, z = (int)(rem - uy * sy);
         const uint8_t tq = type[q];
         const bool inq = tq == NODE_IN;
         T acc[4], acc_state[4] = {0, 0, 0, 0};
@@ -392,18 +391,18 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         }
         // transposed stencils of the rows that read node q: row c = q - e
         // contributes with +, row c = q + e with -
-        // one neighbour per iteration (not unrolled: keeps the loads of one
-        // neighbour in flight together without holding all six in registers)
-#pragma unroll 1
+#pragma unroll
         for (int nb = 0; nb < 6; nb++) {
             const int e = nb >> 1;
             const int sg = (nb & 1) ? 1 : -1;
             const int pe = e == 0 ? x : e == 1 ? y : z;
             const int de = e == 0 ? g.nx : e == 1 ? g.ny : g.nz;
-            const int64_t se = e == 0 ? sx : e == 1 ? sy : 1;
+            const uint32_t se = e == 0 ? sx : e == 1 ? sy : 1u;
             const int pc = pe + sg;
             if (pc < 0 || pc >= de) continue;
-            const int64_t c = q + sg * se;
+            const uint32_t c = sg < 0 ? q - se : q + se;
+            // the stencils below read around c: only a NODE_IN c, never on a
+            // grid face, keeps them inside the grid
             if (type[c] != NODE_IN) continue;
             const T fsg = sg < 0 ? (T)1 : (T)-1;
             const T k = p.t_phi * lam.f[3][c];
