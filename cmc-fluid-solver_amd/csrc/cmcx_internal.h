@@ -93,7 +93,7 @@ void launch_plain_sweep(const Geom& g, int axis, const SegList& segs, const Swee
 // in LDS until the merge, bit 2 = a line spread over 64 lanes (else 32).
 constexpr int kFusedVariants = 10;  // strided sweeps
 constexpr int kZVariants = 8;
-constexpr int kZWaves = 4;  // wavefronts per Z-sweep work-group (two lines each)
+constexpr int kZWaves = 2;  // wavefronts per Z-sweep work-group (two lines each)
 
 // Descriptor of one grid of a batch.  three_dt is passed per launch.
 template <class T>
