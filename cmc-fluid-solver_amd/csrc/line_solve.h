@@ -144,7 +144,8 @@ __device__ __forceinline__ void chunk_reduced(const Chunk<T, M>& ch, const Row<T
 
 // Values of the neighbouring lanes of the wavefront (lane - 1, lane + 1) by
 // DPP wave shifts, which move data in the VALU instead of through the LDS
-// crossbar like __shfl; lane 0 (lane 63) receives 0.
+// crossbar like __shfl; lane 0 (lane 63) receives 0.  Called with every lane
+// of the wavefront active, as __shfl is here.
 __device__ __forceinline__ int lane_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
 __device__ __forceinline__ int lane_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
 __device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev(__float_as_int(x))); }
