@@ -10,7 +10,10 @@ adjoint (cmcx_adjoint_backward): checkpoints every `every` steps, each interval
 recomputed once, transposed line solves in reverse sweep order.
 
 Tensors cross the boundary through host memory, so torch never touches the GPU
-here (the solver owns the device state); gradients come back in float64.
+here (the solver owns the device state); gradients come back in float64.  A
+solver holds one recording: the backward of a rollout raises if another
+rollout was recorded on the same solver since, or its parameters or node
+values were changed (use one solver per live rollout).
 :func:`channel_rollout` builds the inlet-control problem on top of it: node
 values from a coarse inlet profile (each control value fills a block of inlet
 nodes) and the kinematic viscosity, both differentiable.
@@ -40,11 +43,18 @@ class Rollout(torch.autograd.Function):
         solver.adjoint_forward(int(steps), float(dt), int(num_global), int(every))
         u, v, w, _, T = solver.fields()
         ctx.solver = solver
+        ctx.generation = solver.adjoint_generation()
         ctx.devices = (node_values.device, params.device, state0.device)
         return torch.from_numpy(np.stack([u, v, w, T]).astype(np.float64)).to(node_values.device)
 
     @staticmethod
     def backward(ctx, gout):
+        # the solver keeps one recording: another rollout recorded on it since,
+        # or new parameters / node values, would give another rollout's gradient
+        if ctx.solver.adjoint_generation() != ctx.generation:
+            raise RuntimeError("Rollout.backward: the solver's recording is no longer this "
+                               "rollout's (another Rollout.apply, set_params or set_boundary "
+                               "on the same solver since); use one solver per live rollout")
         seed = gout.detach().to("cpu", torch.float64).numpy()
         gb, sb, pb = ctx.solver.adjoint_backward(seed)
         dn, dp, ds = ctx.devices

@@ -235,6 +235,13 @@ class AdiSolver3D:
     def adjoint_release(self):
         check(lib().cmcx_adjoint_release(self.h), "cmcx_adjoint_release")
 
+    def adjoint_generation(self) -> int:
+        """Id of the current recording (0 = none); changes with every
+        recording and whenever a parameter, node-value or option change clears it."""
+        g = ctypes.c_uint64(0)
+        check(lib().cmcx_adjoint_generation(self.h, ctypes.byref(g)), "cmcx_adjoint_generation")
+        return g.value
+
     def set_graph(self, enable: bool = True):
         """Replay each step as a HIP graph (captured on first use)."""
         check(lib().cmcx_set_graph(self.h, 1 if enable else 0), "cmcx_set_graph")

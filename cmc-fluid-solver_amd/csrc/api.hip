@@ -65,6 +65,7 @@ struct ISolver {
     virtual int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
                                  double* pbar) = 0;
     virtual int adjoint_release() = 0;
+    virtual int adjoint_generation(uint64_t* gen) = 0;
     virtual int synchronize() = 0;
 };
 
@@ -101,6 +102,7 @@ class Solver final : public ISolver {
     int nblockz = 1;
     bool built = false;
     double diff_error = 0.0;
+    bool div_blown = false;  // the last evaluation found a grid with a non-finite error
 
     std::vector<uint8_t> h_type, h_bcv, h_bct;  // host copies of the flat mask / BC kinds
     uint8_t *d_type = nullptr, *d_bcv = nullptr, *d_bct = nullptr;
@@ -143,6 +145,10 @@ class Solver final : public ISolver {
     // reverse mode over a recorded rollout
     struct Adj {
         bool recorded = false;
+        // counts recordings and invalidations: a gradient request names the
+        // recording it belongs to (cmcx_adjoint_generation)
+        uint64_t generation = 0;
+        std::string invalid_why;  // what cleared the last recording
         T dt = 0;
         int G = 0, steps = 0, every = 0;
         T* ck = nullptr;       // checkpoints of the state, [nck][4][NT]
@@ -364,6 +370,7 @@ class Solver final : public ISolver {
                     set_error("fused path: grid lines longer than supported");
                     return CMCX_ERR_UNSUPPORTED;
                 }
+        if (p != path || m != merge_order) invalidate_recording("set_options");
         path = p;
         merge_order = m;
         clear_graphs();
@@ -555,8 +562,9 @@ class Solver final : public ISolver {
         ev_used = 0;
     }
 
+    // stores (fused path): bit 0 writes the output, bit 1 the merged Picard layer
     void sweep(int axis, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout, int merges,
-               int var0, int var1, T dt) {
+               int var0, int var1, T dt, int stores = 3) {
         hipEvent_t* ev = nullptr;
         if (prof) {
             if (ev_used >= 4096) harvest_events();
@@ -576,7 +584,7 @@ class Solver final : public ISolver {
             const int v = variant[axis];
             const int lv = axis == AX_Z ? 0 : v;
             launch_fused_sweep<T>(d_desc, (T)3 / dt, axis, work[axis][lv], d_tflags, d_g, in, tin,
-                                  out, tout, merges, bv_synced ? 1 : 0, v, st);
+                                  out, tout, merges, bv_synced ? 1 : 0, stores, v, st);
         }
         if (prof) CMCX_HIP(hipEventRecord(ev[1], st));
     }
@@ -600,7 +608,17 @@ class Solver final : public ISolver {
                     for (int l = 0; l < L; l++) {
                         Layer4<T> tout = other(tin);
                         int merges = (a == AX_X && l == L - 1) ? 2 : 1;
-                        sweep(a, ins[q], tin, outs[q], tout, merges, 0, 4, dt);
+                        // Dead stores the fused sweeps skip (the results do not
+                        // change): an output that the next local sweep overwrites,
+                        // the X output of every Picard iteration but the last (the
+                        // next iteration's Z sweep rewrites it before anything
+                        // reads it; the Y sweep reads it on NODE_IN nodes only,
+                        // which the Z sweep writes), and the Picard layer after
+                        // the step's last sweep (the next step starts from the state).
+                        int stores = 3;
+                        if (l < L - 1 || (a == AX_X && it < G - 1)) stores &= ~1;
+                        if (a == AX_X && it == G - 1 && l == L - 1) stores &= ~2;
+                        sweep(a, ins[q], tin, outs[q], tout, merges, 0, 4, dt, stores);
                         tin = tout;
                     }
                 }
@@ -683,7 +701,11 @@ class Solver final : public ISolver {
         if (compute_err) diff_error = eval_div(next);
         if (err) *err = diff_error;
         if (prof) harvest_events();
-        if (diff_error > 0.01) {
+        // the reference's guard (AdiSolver3D.cpp:371-374) tests err > 0.01, which
+        // a NaN error passes; here a non-finite error of a grid with fluid nodes
+        // (a state that blew up) fires it too.  A grid without fluid nodes
+        // divides 0 by 0 and passes, as in the reference.
+        if (diff_error > 0.01 || div_blown) {
             set_error("divergence error above threshold (0.01)");
             return CMCX_DIVERGED;
         }
@@ -717,16 +739,23 @@ class Solver final : public ISolver {
         CMCX_HIP(hipMemcpyAsync(h.data(), d_acc, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
         CMCX_HIP(hipStreamSynchronize(st));
         double worst = 0.0;
+        div_blown = false;
         for (int i = 0; i < n; i++) {
             grids[i].err = h[2 * i] / h[2 * i + 1];
+            const bool blown = h[2 * i + 1] > 0 && !std::isfinite(grids[i].err);
+            div_blown = div_blown || blown;
             worst = i == 0 ? grids[i].err : std::max(worst, grids[i].err);
         }
+        // a blown-up grid's NaN is the result (std::max drops it after the first)
+        if (div_blown && !std::isinf(worst)) worst = std::nan("");
         return worst;
     }
 
     int div_error(double* err) override {
         if (!err) return CMCX_ERR_ARG;
+        const bool blown = div_blown;  // the step guard's state is the last step's
         eval_div(cur);
+        div_blown = blown;
         *err = G().err;
         return CMCX_OK;
     }
@@ -825,6 +854,7 @@ class Solver final : public ISolver {
                 set_error("BC kind out of range at node " + std::to_string(i));
                 return CMCX_ERR_ARG;
             }
+        invalidate_recording("set_boundary");
         std::memcpy(&h_bcv[o], bv, N);
         std::memcpy(&h_bct[o], bt, N);
         CMCX_HIP(hipMemcpyAsync(d_bcv + o, bv, N, hipMemcpyHostToDevice, st));
@@ -885,7 +915,7 @@ class Solver final : public ISolver {
                     ensure_work(a, lv);
                     auto launch = [&]() {
                         launch_fused_sweep<T>(d_desc, (T)3 / dt, a, work[a][lv], d_tflags, d_g,
-                                              cur, cur, half, tA, 1, bv_synced ? 1 : 0, v, st);
+                                              cur, cur, half, tA, 1, bv_synced ? 1 : 0, 3, v, st);
                     };
                     launch();  // warm-up
                     CMCX_HIP(hipEventRecord(e0, st));
@@ -965,6 +995,7 @@ class Solver final : public ISolver {
     int set_params(const cmcx_params* p) override {
         if (!p) return CMCX_ERR_ARG;
         G().prm = *p;
+        invalidate_recording("set_params");
         upload_desc();
         clear_graphs();
         return CMCX_OK;
@@ -1018,6 +1049,8 @@ class Solver final : public ISolver {
             }
         }
         adj.recorded = true;
+        adj.generation++;
+        adj.invalid_why.clear();
         adj.last_taped = true;
         layers_ok = false;  // record_step leaves next and the Picard layers behind
         return CMCX_OK;
@@ -1099,7 +1132,10 @@ class Solver final : public ISolver {
     int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
                          double* pbar) override {
         if (!adj.recorded) {
-            set_error("adjoint_backward before adjoint_forward");
+            set_error(adj.invalid_why.empty()
+                          ? std::string("adjoint_backward before adjoint_forward")
+                          : "the recorded rollout was invalidated by " + adj.invalid_why +
+                                " (record it again with adjoint_forward)");
             return CMCX_ERR_STATE;
         }
         const Geom& g = grids[0].g;
@@ -1157,6 +1193,22 @@ class Solver final : public ISolver {
         return CMCX_OK;
     }
 
+    // The recompute and reverse of a recorded rollout read the solver's
+    // parameters, node values, BC kinds and path: changing one of them leaves a
+    // recording whose gradient would belong to another rollout.
+    void invalidate_recording(const char* why) {
+        if (!adj.recorded) return;
+        adj.recorded = false;
+        adj.generation++;
+        adj.invalid_why = why;
+    }
+
+    int adjoint_generation(uint64_t* gen) override {
+        if (!gen) return CMCX_ERR_ARG;
+        *gen = adj.recorded ? adj.generation : 0;
+        return CMCX_OK;
+    }
+
     int adjoint_release() override {
         hipStreamSynchronize(st);
         hipFree(adj.ck);
@@ -1165,7 +1217,7 @@ class Solver final : public ISolver {
         adj.tape = nullptr;
         adj.nck = 0;
         adj.tape_layers = 0;
-        adj.recorded = false;
+        invalidate_recording("adjoint_release");
         adj.last_taped = false;
         return CMCX_OK;
     }
@@ -1357,6 +1409,7 @@ int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* 
     SOLVER_CALL(s, adjoint_backward(seed, gbar, sbar, pbar));
 }
 int cmcx_adjoint_release(cmcx_solver* s) { SOLVER_CALL(s, adjoint_release()); }
+int cmcx_adjoint_generation(cmcx_solver* s, uint64_t* gen) { SOLVER_CALL(s, adjoint_generation(gen)); }
 int cmcx_synchronize(cmcx_solver* s) { SOLVER_CALL(s, synchronize()); }
 
 }  // extern "C"

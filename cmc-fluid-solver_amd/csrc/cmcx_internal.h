@@ -126,12 +126,13 @@ constexpr uint8_t kFlagFreeVel = 4, kFlagFreeT = 8;
 void build_node_flags(const uint8_t* t, const uint8_t* bv, const uint8_t* bt, int64_t n,
                       uint8_t* out);
 // bsync: the state holds the node values on BOUND/VALVE nodes with no-slip BCs
-// (the Picard layer then supplies the Dirichlet values).
+// (the Picard layer then supplies the Dirichlet values).  stores: bit 0 writes
+// the sweep output, bit 1 the merged Picard layer (3 = both).
 template <class T>
 void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
                         const uint8_t* tflags, const T* const* gval, Layer4<T> in, Layer4<T> tin,
-                        Layer4<T> out, Layer4<T> tout, int merges, int bsync, int variant,
-                        hipStream_t st);
+                        Layer4<T> out, Layer4<T> tout, int merges, int bsync, int stores,
+                        int variant, hipStream_t st);
 // Builds the work of `axis` from the host copy of the flat node types; lpg: z
 // lines per tile of the strided (X, Y) sweeps; elem: sizeof the field type.
 void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,

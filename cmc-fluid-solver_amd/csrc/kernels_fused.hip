@@ -62,19 +62,34 @@ namespace {
 
 using namespace lsolve;
 
-// Interior row at node c (stride s along the line): both matrices and the four
-// right-hand sides, with the pressure term on the field along the sweep and
-// the dissipation function Phi_d on T (TimeLayer3D.h:338-343, 554-588).
+// Correction form.  Every sweep solves for the correction delta = x - t to
+// the Picard layer t (the current iterate) instead of for x itself: A delta =
+// d - A t, with t on every row of a system.  For an interior row (b = 3/dt +
+// 2 nu/h^2, a + c = -2 nu/h^2) the right-hand side is
+//   3/dt (x_in - t_j) + adv (t_{j-1} - t_{j+1}) + nu/h^2 (t_{j-1} + t_{j+1} - 2 t_j) + e_j,
+// e_j the pressure term or v_T-scaled T gradient of the field along the sweep,
+// or t_phi Phi_d for T: the same discrete equations as the reference's rows
+// (AdiSolver3D.cpp:755-801), with every term a difference of neighbouring
+// values.  Solving for x directly rounds the right-hand side 3/dt x_in and
+// the whole solution at the scale of x, and in FP32 that hides the small
+// components: v and w beside u, and T - baseT beside T ~ baseT, which the
+// correction keeps at their own scale.  Boundary rows: a Dirichlet row takes
+// delta = value - t (0 while the state is in sync: the Picard layer holds
+// the node value); a free row (2 x_j - x_{j+-1} = 0) takes t_{j+-1} - 2 t_j.
+// The sweep writes x = t + delta (the value itself on Dirichlet rows) and the
+// merged Picard layer t + delta/2 (t + 3 delta/4 after the folded second
+// merge).
+
+// Interior row in correction form: wm, w0, wp are the Picard layer at the rows
+// before, at and after the node (per field), cin the sweep input at the node,
+// ca and cb the central differences (not yet scaled) of the cross-line
+// derivatives of Phi_d (X: u along y and z; Y: v along x and z; Z: w along x
+// and y) (TimeLayer3D.h:338-343, 554-588).
 template <class T, int AXIS>
-__device__ __forceinline__ void interior_row(const Geom& g, const SweepParams<T>& p, int64_t c,
-                                             int64_t s, const Layer4<T>& in, const Layer4<T>& tin,
-                                             Row<T>& r) {
-    const T* tu = tin.f[0];
-    const T* tv = tin.f[1];
-    const T* tw = tin.f[2];
-    const T* tt = tin.f[3];
-    const T w = tin.f[AXIS][c];
-    const T adv = w * p.inv_2h;
+__device__ __forceinline__ void interior_row_delta(const SweepParams<T>& p, const T* wm,
+                                                   const T* w0, const T* wp, const T* cin, T ca,
+                                                   T cb, Row<T>& r) {
+    const T adv = w0[AXIS] * p.inv_2h;
     r.av = -adv - p.vis_v;
     r.cv = adv - p.vis_v;
     r.bv = p.three_dt + 2 * p.vis_v;
@@ -82,81 +97,147 @@ __device__ __forceinline__ void interior_row(const Geom& g, const SweepParams<T>
     r.ct = adv - p.vis_t;
     r.bt = p.three_dt + 2 * p.vis_t;
 #pragma unroll
-    for (int v = 0; v < 4; v++) r.d[v] = in.f[v][c] * p.three_dt;
-    const T grad = (tt[c + s] - tt[c - s]) * p.inv_2h;
-    r.d[AXIS] = r.d[AXIS] - p.v_T * grad;
+    for (int v = 0; v < 4; v++) {
+        const T vis = v < 3 ? p.vis_v : p.vis_t;
+        r.d[v] = p.three_dt * (cin[v] - w0[v]) + adv * (wm[v] - wp[v]) +
+                 vis * ((wm[v] + wp[v]) - 2 * w0[v]);
+    }
+    r.d[AXIS] = r.d[AXIS] - p.v_T * ((wp[3] - wm[3]) * p.inv_2h);
     T phi;
     if (AXIS == AX_X) {
-        T u_x = (tu[c + s] - tu[c - s]) * p.inv_2hx;
-        T v_x = (tv[c + s] - tv[c - s]) * p.inv_2hx;
-        T w_x = (tw[c + s] - tw[c - s]) * p.inv_2hx;
-        T u_y = (tu[c + g.sy] - tu[c - g.sy]) * p.inv_2hy;
-        T u_z = (tu[c + 1] - tu[c - 1]) * p.inv_2hz;
+        const T u_x = (wp[0] - wm[0]) * p.inv_2hx, v_x = (wp[1] - wm[1]) * p.inv_2hx;
+        const T w_x = (wp[2] - wm[2]) * p.inv_2hx;
+        const T u_y = ca * p.inv_2hy, u_z = cb * p.inv_2hz;
         phi = 2 * u_x * u_x + v_x * v_x + w_x * w_x + v_x * u_y + w_x * u_z;
     } else if (AXIS == AX_Y) {
-        T u_y = (tu[c + s] - tu[c - s]) * p.inv_2hy;
-        T v_y = (tv[c + s] - tv[c - s]) * p.inv_2hy;
-        T w_y = (tw[c + s] - tw[c - s]) * p.inv_2hy;
-        T v_x = (tv[c + g.sx] - tv[c - g.sx]) * p.inv_2hx;
-        T v_z = (tv[c + 1] - tv[c - 1]) * p.inv_2hz;
+        const T u_y = (wp[0] - wm[0]) * p.inv_2hy, v_y = (wp[1] - wm[1]) * p.inv_2hy;
+        const T w_y = (wp[2] - wm[2]) * p.inv_2hy;
+        const T v_x = ca * p.inv_2hx, v_z = cb * p.inv_2hz;
         phi = u_y * u_y + 2 * v_y * v_y + w_y * w_y + u_y * v_x + w_y * v_z;
     } else {
-        T u_z = (tu[c + s] - tu[c - s]) * p.inv_2hz;
-        T v_z = (tv[c + s] - tv[c - s]) * p.inv_2hz;
-        T w_z = (tw[c + s] - tw[c - s]) * p.inv_2hz;
-        T w_x = (tw[c + g.sx] - tw[c - g.sx]) * p.inv_2hx;
-        T w_y = (tw[c + g.sy] - tw[c - g.sy]) * p.inv_2hy;
+        const T u_z = (wp[0] - wm[0]) * p.inv_2hz, v_z = (wp[1] - wm[1]) * p.inv_2hz;
+        const T w_z = (wp[2] - wm[2]) * p.inv_2hz;
+        const T w_x = ca * p.inv_2hx, w_y = cb * p.inv_2hy;
         phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * w_x + v_z * w_y;
     }
     r.d[3] = r.d[3] + p.t_phi * phi;
 }
 
-// Boundary or identity row (ApplyBC0/ApplyBC1) from the node's flags byte (BC
-// kinds in bits 2 and 3); a Dirichlet row takes its value from src[v][c].
+// Boundary or identity row (ApplyBC0 / ApplyBC1, AdiSolver3D.cpp:804-852) in
+// correction form, without branches: tf is the node's flags byte (BC kinds in
+// bits 2 and 3), gv the Dirichlet values.
 template <class T>
-__device__ __forceinline__ void boundary_row(int role, uint8_t tf, int64_t c, const T* const* src,
-                                             Row<T>& r) {
-    bool uv, ut;
-    bc_row<T>(role, role != RL_NONE && (tf & kFlagFreeVel), r.av, r.bv, r.cv, uv);
-    bc_row<T>(role, role != RL_NONE && (tf & kFlagFreeT), r.at, r.bt, r.ct, ut);
+__device__ __forceinline__ void boundary_row_delta(int role, uint8_t tf, const T* wm, const T* w0,
+                                                   const T* wp, const T* gv, Row<T>& r) {
+    const bool fsv = role == RL_START && (tf & kFlagFreeVel), fev = role == RL_END && (tf & kFlagFreeVel);
+    const bool fst = role == RL_START && (tf & kFlagFreeT), fet = role == RL_END && (tf & kFlagFreeT);
+    const bool val = role != RL_NONE;
+    r.av = fev ? (T)-1 : (T)0;
+    r.bv = (fsv || fev) ? (T)2 : (T)1;
+    r.cv = fsv ? (T)-1 : (T)0;
+    r.at = fet ? (T)-1 : (T)0;
+    r.bt = (fst || fet) ? (T)2 : (T)1;
+    r.ct = fst ? (T)-1 : (T)0;
 #pragma unroll
-    for (int v = 0; v < 3; v++) r.d[v] = uv ? src[v][c] : (T)0;
-    r.d[3] = ut ? src[3][c] : (T)0;
+    for (int v = 0; v < 4; v++) {
+        const bool fs = v < 3 ? fsv : fst, fe = v < 3 ? fev : fet;
+        const T dir = val ? gv[v] - w0[v] : (T)0;
+        r.d[v] = fs ? wp[v] - 2 * w0[v] : fe ? wm[v] - 2 * w0[v] : dir;
+    }
 }
 
-// Loads the chunk's rows, eliminates them and returns the two reduced rows.
-// Row r of the line lives at base + r * s; rows outside [0, n) or !ok are
-// identity rows.  role[] receives the row roles.
-template <class T, int AXIS, int M, bool KEEP>
-__device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>& p,
-                                            const uint8_t* __restrict__ type, const T* const* bsrc,
-                                            const Layer4<T>& in, const Layer4<T>& tin,
-                                            int64_t base, int64_t s, int r0, int n, bool ok,
-                                            int* role, Chunk<T, M>& ch, T* f, T* l, T* keep,
-                                            int kfs, int krs) {
+// Whether row `role` of matrix q (0 velocity, 1 T) is a Dirichlet row.
+__device__ __forceinline__ bool dirichlet_row(int role, uint8_t tf, int q) {
+    const uint8_t fb = q == 0 ? kFlagFreeVel : kFlagFreeT;
+    return role != RL_NONE && role != RL_IN && !((role == RL_START || role == RL_END) && (tf & fb));
+}
+
+// Strided chunk: loads the chunk's rows (and the Picard layer one row either
+// side), eliminates them and returns the two reduced rows.  Row r of the line
+// lives at element base + r * s; rows outside [0, n), and every row when !ok,
+// read the interior node `safe` instead (so that each load is unconditional
+// and in bounds) and are identity rows.  The loads are software-pipelined:
+// those of row j + LA are issued before row j is eliminated, in one basic
+// block, so that LA + 1 rows are in flight instead of one (a row whose loads
+// sit in a branch of its own waits for all of them before the next row's
+// loads can issue).  role[] receives the row roles.  KEEP parks the chunk's
+// Picard layer in LDS for the write phase.
+template <class T, int AXIS, int M, bool KEEP, bool BSYNC>
+__device__ __forceinline__ void strided_chunk(const Geom& g, const SweepParams<T>& p,
+                                              const uint8_t* __restrict__ type,
+                                              const T* const* gval, const Layer4<T>& in,
+                                              const Layer4<T>& tin, int base, int s, int safe,
+                                              int r0, int n, bool ok, int* role, Chunk<T, M>& ch,
+                                              T* f, T* l, T* keep, int kfs, int krs) {
+    constexpr int LA = 2;  // rows loaded ahead of the elimination
+    auto addr = [&](int i) {  // element of window row i (line row r0 - 1 + i)
+        const int r = r0 + i - 1;
+        return (ok && r >= 0 && r < n) ? base + r * s : safe;
+    };
+    auto valid = [&](int i) {
+        const int r = r0 + i - 1;
+        return ok && r >= 0 && r < n;
+    };
     uint8_t ty[M + 2];
 #pragma unroll
-    for (int j = 0; j < M + 2; j++) {
-        int r = r0 + j - 1;
-        ty[j] = (ok && r >= 0 && r < n) ? type[base + (int64_t)r * s] : kOutside;
+    for (int i = 0; i < M + 2; i++) {
+        const uint8_t t = type[addr(i)];
+        ty[i] = valid(i) ? t : kOutside;
     }
+    constexpr int XF = AXIS == AX_X ? 0 : 1;  // field of the cross-line derivatives
+    const int sa = AXIS == AX_X ? (int)g.sy : (int)g.sx;
+    const T* xfp = tin.f[XF];
+    // Picard layer window (rows r0-1 .. r0+M), sweep input and cross-line
+    // differences, filled LA rows ahead of their use
+    T w[4][M + 2], cin[4][M], ca[M], cb[M];
+    auto load_row = [&](int j) {  // window row j + 1 and row j's own values
+        const int c = addr(j + 1);
+#pragma unroll
+        for (int v = 0; v < 4; v++) w[v][j + 1] = tin.f[v][c];
+#pragma unroll
+        for (int v = 0; v < 4; v++) cin[v][j] = in.f[v][c];
+        ca[j] = xfp[c + sa] - xfp[c - sa];
+        cb[j] = xfp[c + 1] - xfp[c - 1];
+    };
+    {
+        const int c = addr(0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) w[v][0] = tin.f[v][c];
+    }
+#pragma unroll
+    for (int j = 0; j < LA && j < M; j++) load_row(j);
 #pragma unroll
     for (int j = 0; j < M; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
     Row<T> rfirst, rlast;
 #pragma unroll
     for (int j = 0; j < M; j++) {
-        const int64_t c = base + (int64_t)(r0 + j) * s;
-        Row<T> rw;
-        if (role[j] == RL_IN) {
-            interior_row<T, AXIS>(g, p, c, s, in, tin, rw);
-            if (KEEP) {
-                // the Picard layer of this row, for the merge of the write phase
+        if (j + LA < M) load_row(j + LA);
+        if (j + LA == M - 1 + 1 || (j == M - 1 && LA == 0)) {
+            const int c = addr(M + 1);
 #pragma unroll
-                for (int v = 0; v < 4; v++) keep[v * kfs + j * krs] = tin.f[v][c];
-            }
-        } else {
-            boundary_row<T>(role[j], ty[j + 1], c, bsrc, rw);
+            for (int v = 0; v < 4; v++) w[v][M + 1] = tin.f[v][c];
         }
+        T wm[4], w0[4], wp[4], ci[4], gd[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            wm[v] = w[v][j];
+            w0[v] = w[v][j + 1];
+            wp[v] = w[v][j + 2];
+            ci[v] = cin[v][j];
+            gd[v] = BSYNC ? w0[v] : gval[v][addr(j + 1)];
+        }
+        if (KEEP) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) keep[v * kfs + j * krs] = w0[v];
+        }
+        Row<T> ri, rb, rw;
+        interior_row_delta<T, AXIS>(p, wm, w0, wp, ci, ca[j], cb[j], ri);
+        boundary_row_delta<T>(role[j], ty[j + 1], wm, w0, wp, gd, rb);
+        const bool isin = role[j] == RL_IN;
+        rw.av = isin ? ri.av : rb.av; rw.bv = isin ? ri.bv : rb.bv; rw.cv = isin ? ri.cv : rb.cv;
+        rw.at = isin ? ri.at : rb.at; rw.bt = isin ? ri.bt : rb.bt; rw.ct = isin ? ri.ct : rb.ct;
+#pragma unroll
+        for (int v = 0; v < 4; v++) rw.d[v] = isin ? ri.d[v] : rb.d[v];
         if (j == 0) rfirst = rw;
         else if (j == M - 1) rlast = rw;
         else chunk_down<T, M>(ch, j, rw);
@@ -165,34 +246,39 @@ __device__ __forceinline__ void chunk_phase(const Geom& g, const SweepParams<T>&
     chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
 }
 
-// Writes the chunk's rows given x_first / x_last of the four fields.
-template <class T, int M, bool KEEP>
-__device__ __forceinline__ void write_phase(const Chunk<T, M>& ch, const int* role, int64_t base,
-                                            int64_t s, int r0, const T* xf, const T* xl,
-                                            const Layer4<T>& tin, const Layer4<T>& out,
-                                            const Layer4<T>& tout, int merges, const T* keep,
-                                            int kfs, int krs) {
+// Writes the chunk's rows given the corrections df / dl of its first and last
+// rows: x = t + delta on every row of a system (the node value on Dirichlet
+// rows), the merged Picard layer on NODE_IN rows.  Row j lives at element
+// base + (r0 + j) * s.
+template <class T, int M, bool KEEP, bool BSYNC>
+__device__ __forceinline__ void strided_write(const Chunk<T, M>& ch, const int* role,
+                                              const uint8_t* __restrict__ type, int base, int s,
+                                              int r0, const T* df, const T* dl,
+                                              const T* const* gval, const Layer4<T>& tin,
+                                              const Layer4<T>& out, const Layer4<T>& tout,
+                                              int merges, int stores, const T* keep, int kfs,
+                                              int krs) {
+    const T km = merges > 1 ? (T)0.75 : (T)0.5;
 #pragma unroll
     for (int j = 0; j < M; j++) {
         if (role[j] == RL_NONE) continue;
-        const int64_t c = base + (int64_t)(r0 + j) * s;
+        const int c = base + (r0 + j) * s;
+        const uint8_t tf = BSYNC ? (uint8_t)0 : type[c];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            T x;
-            if (j == 0) x = xf[v];
-            else if (j == M - 1) x = xl[v];
+            T d;
+            if (j == 0) d = df[v];
+            else if (j == M - 1) d = dl[v];
             else {
-                T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
-                T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                x = ch.E[v][j] + F * xf[v] + G * xl[v];
+                const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
+                const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
+                d = ch.E[v][j] + F * df[v] + G * dl[v];
             }
-            out.f[v][c] = x;
-            if (role[j] == RL_IN) {
-                const T t0 = KEEP ? keep[v * kfs + j * krs] : tin.f[v][c];
-                T m = (t0 + x) * (T)0.5;
-                if (merges > 1) m = (m + x) * (T)0.5;
-                tout.f[v][c] = m;
-            }
+            const T t0 = KEEP ? keep[v * kfs + j * krs] : tin.f[v][c];
+            T x = t0 + d;
+            if (!BSYNC && dirichlet_row(role[j], tf, v < 3 ? 0 : 1)) x = gval[v][c];
+            if (stores & 1) out.f[v][c] = x;
+            if ((stores & 2) && role[j] == RL_IN) tout.f[v][c] = t0 + km * d;
         }
     }
 }
@@ -213,23 +299,15 @@ __host__ __device__ constexpr int keep_row_pitch(int lpg, int m) {
 // every row of them (MAXT = 32 * LPG threads for lines of up to 256 rows in
 // chunks of M = 8); MINW is the occupancy the register allocation must allow
 // (waves per SIMD), so that two half-width work-groups can share a CU.
-template <class T, int AXIS, int M, int LPG, int MAXT, int MINW, bool KEEP>
-__global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
-    const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
-    const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
-    const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
-    int merges, int bsync, const int32_t* __restrict__ work) {
+template <class T, int AXIS, int M, int LPG, bool KEEP, bool BSYNC>
+__device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ insts, T three_dt,
+                                             const uint8_t* __restrict__ type, const T* const* gval,
+                                             const Layer4<T>& in, const Layer4<T>& tin,
+                                             const Layer4<T>& out, const Layer4<T>& tout,
+                                             int merges, int stores,
+                                             const int32_t* __restrict__ work) {
     extern __shared__ char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);
-    // Dirichlet values of the boundary rows: the Picard layer while the state is
-    // in sync with the node values (it holds them off NODE_IN), else the node values
-    const T* bsrc[4];
-    if (bsync) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) bsrc[v] = tin.f[v];
-    } else {
-        bsrc[0] = gv0; bsrc[1] = gv1; bsrc[2] = gv2; bsrc[3] = gv3;
-    }
     constexpr int GPW = 64 / LPG;  // chunks (lane groups) per wave
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -246,13 +324,17 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     SweepParams<T> p = insts[inst].p[AXIS];
     p.three_dt = three_dt;
     const int n = AXIS == AX_X ? g.nx : g.ny;
-    const int64_t s = AXIS == AX_X ? g.sx : g.sy;
+    const int s = AXIS == AX_X ? (int)g.sx : (int)g.sy;
     const int ztiles = (g.nz + LPG - 1) / LPG;
     const int outer = tile / ztiles;
     const int z = (tile % ztiles) * LPG + gl;
     const bool zok = z < g.nz;
-    const int64_t base =
-        insts[inst].off + (AXIS == AX_X ? (int64_t)outer * g.sy : (int64_t)outer * g.sx) + z;
+    // a grid of a batch holds fewer than 2^31 nodes with its offset (checked at
+    // creation): 32-bit element indices
+    const int off = (int)insts[inst].off;
+    const int base = off + (AXIS == AX_X ? outer * (int)g.sy : outer * (int)g.sx) + z;
+    // interior node (1, 1, 1): the element read by rows outside the line
+    const int safe = off + (int)g.sx + (int)g.sy + 1;
     const int r0 = chunk * M;
 
     // KEEP: each lane parks the Picard layer of its rows in LDS, after the
@@ -268,8 +350,8 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     int role[M];
     Chunk<T, M> ch;
     T f[kRedFields], l[kRedFields];
-    chunk_phase<T, AXIS, M, KEEP>(g, p, type, bsrc, in, tin, base, s, r0, n, zok, role, ch, f, l,
-                                  keep, kfs, krs);
+    strided_chunk<T, AXIS, M, KEEP, BSYNC>(g, p, type, gval, in, tin, base, s, safe, r0, n, zok,
+                                           role, ch, f, l, keep, kfs, krs);
     T xf[4], xl[4];
     if constexpr (KEEP) {
         const NormLayout NL{pow2_at_least(P)};
@@ -315,7 +397,30 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
             xl[v] = red[L.at(gl, 6 + v, 2 * chunk + 1)];
         }
     }
-    write_phase<T, M, KEEP>(ch, role, base, s, r0, xf, xl, tin, out, tout, merges, keep, kfs, krs);
+    strided_write<T, M, KEEP, BSYNC>(ch, role, type, base, s, r0, xf, xl, gval, tin, out, tout,
+                                     merges, stores, keep, kfs, krs);
+}
+
+// ---------------------------------------------------------------------------
+// Strided lines (X or Y sweep).  A work-group owns LPG consecutive z lines and
+// every row of them (MAXT = 32 * LPG threads for lines of up to 256 rows in
+// chunks of M = 8); MINW is the occupancy the register allocation must allow
+// (waves per SIMD), so that two half-width work-groups can share a CU.
+// BSYNC: the state holds the node values on BOUND/VALVE nodes with no-slip
+// BCs, so the Dirichlet values come from the Picard layer (else from the node
+// values; that case, only met by a step without a boundary fill after the
+// caller replaced the state or node values, has one instantiation per tile
+// shape, without the LDS-parked layer); stores: bit 0 writes the sweep output, bit 1 the merged Picard
+// layer (a sweep whose output or Picard layer nothing reads skips it).
+template <class T, int AXIS, int M, int LPG, int MAXT, int MINW, bool KEEP, bool BSYNC>
+__global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
+    const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
+    const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
+    const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
+    int merges, int stores, const int32_t* __restrict__ work) {
+    const T* gval[4] = {gv0, gv1, gv2, gv3};
+    strided_tile<T, AXIS, M, LPG, KEEP, BSYNC>(insts, three_dt, type, gval, in, tin, out, tout,
+                                               merges, stores, work);
 }
 
 // Contiguous lines (Z sweep): 32 lanes per line, two lines per wavefront.  A
@@ -333,8 +438,8 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
     const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
     const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
-    int merges, int bsync, const int32_t* __restrict__ lines, const int32_t* __restrict__ winst,
-    int vec_ok) {
+    int merges, int bsync, int stores, const int32_t* __restrict__ lines,
+    const int32_t* __restrict__ winst, int vec_ok) {
     extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
@@ -406,11 +511,11 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
         load_run<T, M>(tw + c0 + (ok ? g.sx : 0), a, vec, valid);
         load_run<T, M>(tw + c0 - (ok ? g.sx : 0), b, vec, valid);
 #pragma unroll
-        for (int j = 0; j < M; j++) wx[j] = (a[j] - b[j]) * p.inv_2hx;
+        for (int j = 0; j < M; j++) wx[j] = a[j] - b[j];
         load_run<T, M>(tw + c0 + (ok ? g.sy : 0), a, vec, valid);
         load_run<T, M>(tw + c0 - (ok ? g.sy : 0), b, vec, valid);
 #pragma unroll
-        for (int j = 0; j < M; j++) wy[j] = (a[j] - b[j]) * p.inv_2hy;
+        for (int j = 0; j < M; j++) wy[j] = a[j] - b[j];
     }
     T tprev[4], tnext[4];
 #pragma unroll
@@ -439,37 +544,24 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     Row<T> rfirst, rlast;
 #pragma unroll
     for (int j = 0; j < M; j++) {
-        Row<T> r;
-        if (role[j] == RL_IN) {
-            const T adv = tv[2][j] * p.inv_2h;
-            r.av = -adv - p.vis_v;
-            r.cv = adv - p.vis_v;
-            r.bv = p.three_dt + 2 * p.vis_v;
-            r.at = -adv - p.vis_t;
-            r.ct = adv - p.vis_t;
-            r.bt = p.three_dt + 2 * p.vis_t;
-            T m1[4], p1[4];
+        // correction form (see interior_row_delta): Picard layer around the row
+        T wm[4], w0[4], wp[4], ci[4], gd[4];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                m1[v] = j > 0 ? tv[v][j - 1] : tprev[v];
-                p1[v] = j < M - 1 ? tv[v][j + 1] : tnext[v];
-                r.d[v] = cu[v][j] * p.three_dt;
-            }
-            r.d[2] = r.d[2] - p.v_T * ((p1[3] - m1[3]) * p.inv_2h);
-            const T u_z = (p1[0] - m1[0]) * p.inv_2hz;
-            const T v_z = (p1[1] - m1[1]) * p.inv_2hz;
-            const T w_z = (p1[2] - m1[2]) * p.inv_2hz;
-            const T phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * wx[j] + v_z * wy[j];
-            r.d[3] = r.d[3] + p.t_phi * phi;
-        } else {
-            const uint8_t tf = ty[j + 1];
-            bool uv, ut;
-            bc_row<T>(role[j], role[j] != RL_NONE && (tf & kFlagFreeVel), r.av, r.bv, r.cv, uv);
-            bc_row<T>(role[j], role[j] != RL_NONE && (tf & kFlagFreeT), r.at, r.bt, r.ct, ut);
-#pragma unroll
-            for (int v = 0; v < 3; v++) r.d[v] = uv ? (bsync ? tv[v][j] : gb[v][j]) : (T)0;
-            r.d[3] = ut ? (bsync ? tv[3][j] : gb[3][j]) : (T)0;
+        for (int v = 0; v < 4; v++) {
+            wm[v] = j > 0 ? tv[v][j - 1] : tprev[v];
+            w0[v] = tv[v][j];
+            wp[v] = j < M - 1 ? tv[v][j + 1] : tnext[v];
+            ci[v] = cu[v][j];
+            gd[v] = bsync ? w0[v] : gb[v][j];
         }
+        Row<T> ri, rb, r;
+        interior_row_delta<T, AX_Z>(p, wm, w0, wp, ci, wx[j], wy[j], ri);
+        boundary_row_delta<T>(role[j], ty[j + 1], wm, w0, wp, gd, rb);
+        const bool isin = role[j] == RL_IN;
+        r.av = isin ? ri.av : rb.av; r.bv = isin ? ri.bv : rb.bv; r.cv = isin ? ri.cv : rb.cv;
+        r.at = isin ? ri.at : rb.at; r.bt = isin ? ri.bt : rb.bt; r.ct = isin ? ri.ct : rb.ct;
+#pragma unroll
+        for (int v = 0; v < 4; v++) r.d[v] = isin ? ri.d[v] : rb.d[v];
         if (j == 0) rfirst = r;
         else if (j == M - 1) rlast = r;
         else chunk_down<T, M>(ch, j, r);
@@ -498,27 +590,29 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
                     if (k * VW + q < M) tv[v][k * VW + q] = e[q];
             }
     }
+    const T km = merges > 1 ? (T)0.75 : (T)0.5;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         T x[M], t[M];
 #pragma unroll
         for (int j = 0; j < M; j++) {
-            if (j == 0) x[j] = xf[v];
-            else if (j == M - 1) x[j] = xl[v];
+            T d;  // correction of row j (xf / xl: of the chunk's first / last row)
+            if (j == 0) d = xf[v];
+            else if (j == M - 1) d = xl[v];
             else {
                 const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
                 const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                x[j] = ch.E[v][j] + F * xf[v] + G * xl[v];
+                d = ch.E[v][j] + F * xf[v] + G * xl[v];
             }
+            x[j] = tv[v][j] + d;
+            if (!bsync && dirichlet_row(role[j], ty[j + 1], v < 3 ? 0 : 1)) x[j] = gb[v][j];
             // rows no segment covers get the sweep input's value
             if (role[j] == RL_NONE) x[j] = cu[v][j];
-            T m = (tv[v][j] + x[j]) * (T)0.5;
-            if (merges > 1) m = (m + x[j]) * (T)0.5;
-            t[j] = role[j] == RL_IN ? m : tv[v][j];
+            t[j] = role[j] == RL_IN ? tv[v][j] + km * d : tv[v][j];
         }
         // whole runs: partial-vector stores would cost far more than the bytes
-        store_run<T, M>(out.f[v] + c0, x, vec, valid);
-        store_run<T, M>(tout.f[v] + c0, t, vec, valid);
+        if (stores & 1) store_run<T, M>(out.f[v] + c0, x, vec, valid);
+        if (stores & 2) store_run<T, M>(tout.f[v] + c0, t, vec, valid);
     }
 }
 
@@ -600,8 +694,8 @@ bool fused_variant_ok(int n, int v) {
 template <class T>
 void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const FusedWork& w,
                         const uint8_t* tflags, const T* const* gval, Layer4<T> in, Layer4<T> tin,
-                        Layer4<T> out, Layer4<T> tout, int merges, int bsync, int variant,
-                        hipStream_t st) {
+                        Layer4<T> out, Layer4<T> tout, int merges, int bsync, int stores,
+                        int variant, hipStream_t st) {
     if (w.nblocks == 0) return;
     const unsigned blocks = (unsigned)w.nblocks;
     if (axis == AX_Z) {
@@ -619,7 +713,7 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
 #define LZ(MM, W, PK, LL)                                                                          \
     hipLaunchKernelGGL((fused_z_kernel<T, MM, W, PK, LL>), dim3(blocks), dim3(nthreads), lds, st, \
                        insts, three_dt, tflags, gval[0], gval[1], gval[2], gval[3], in, tin, out,   \
-                       tout, merges, bsync, w.d_items, w.d_inst, w.vec_ok)
+                       tout, merges, bsync, stores, w.d_items, w.d_inst, w.vec_ok)
 #define LZM(W, PK)                      \
     if (!wide) {                        \
         if (m == 2) LZ(2, W, PK, 32);   \
@@ -649,25 +743,40 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
             throw std::string("fused sweep: tile variant does not fit the line length");
         const int waves = strided_waves_for(w.maxn, lpg, mch);
         size_t lds = strided_lds<T>(w.maxn, variant);
-#define LS(AX, V)                                                                                 \
+#define LS(AX, V, BS)                                                                             \
     hipLaunchKernelGGL((fused_strided_kernel<T, AX, kShapes[V].chunk, kShapes[V].bytes / (int)sizeof(T), \
                                              32 * (kShapes[V].bytes / (int)sizeof(T)),            \
-                                             16 / (int)sizeof(T), kShapes[V].keep>),              \
+                                             16 / (int)sizeof(T), kShapes[V].keep, BS>),          \
                        dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, tflags, gval[0], \
-                       gval[1], gval[2], gval[3], in, tin, out, tout, merges, bsync, w.d_items)
-#define LSA(AX)                        \
-    switch (variant) {                 \
-        case 0: LS(AX, 0); break;      \
-        case 1: LS(AX, 1); break;      \
-        case 2: LS(AX, 2); break;      \
-        case 3: LS(AX, 3); break;      \
-        case 4: LS(AX, 4); break;      \
-        case 5: LS(AX, 5); break;      \
-        case 6: LS(AX, 6); break;      \
-        case 7: LS(AX, 7); break;      \
-        case 8: LS(AX, 8); break;      \
-        default: LS(AX, 9); break;     \
+                       gval[1], gval[2], gval[3], in, tin, out, tout, merges, stores, w.d_items)
+#define LSA(AX)                                 \
+    if (bsync) {                                \
+        switch (variant) {                      \
+            case 0: LS(AX, 0, true); break;     \
+            case 1: LS(AX, 1, true); break;     \
+            case 2: LS(AX, 2, true); break;     \
+            case 3: LS(AX, 3, true); break;     \
+            case 4: LS(AX, 4, true); break;     \
+            case 5: LS(AX, 5, true); break;     \
+            case 6: LS(AX, 6, true); break;     \
+            case 7: LS(AX, 7, true); break;     \
+            case 8: LS(AX, 8, true); break;     \
+            default: LS(AX, 9, true); break;    \
+        }                                       \
+    } else {                                    \
+        switch (oos) {                          \
+            case 0: LS(AX, 0, false); break;    \
+            case 1: LS(AX, 1, false); break;    \
+            case 2: LS(AX, 2, false); break;    \
+            case 3: LS(AX, 3, false); break;    \
+            case 4: LS(AX, 4, false); break;    \
+            default: LS(AX, 5, false); break;   \
+        }                                       \
     }
+        // out of sync: the variant of the same tile shape without the parked
+        // layer (the same work list: it depends on the tile width only)
+        const int oos = kShapes[variant].keep ? variant - 6 : variant;
+        if (!bsync) lds = strided_lds<T>(w.maxn, oos);
         if (axis == AX_X) {
             LSA(AX_X);
         } else {
@@ -792,10 +901,10 @@ template int strided_lpg<double>(int);
 template void launch_fused_sweep<float>(const InstDesc<float>*, float, int, const FusedWork&,
                                         const uint8_t*, const float* const*, Layer4<float>,
                                         Layer4<float>, Layer4<float>, Layer4<float>, int, int, int,
-                                        hipStream_t);
+                                        int, hipStream_t);
 template void launch_fused_sweep<double>(const InstDesc<double>*, double, int, const FusedWork&,
                                          const uint8_t*, const double* const*, Layer4<double>,
                                          Layer4<double>, Layer4<double>, Layer4<double>, int, int,
-                                         int, hipStream_t);
+                                         int, int, hipStream_t);
 
 }  // namespace cmcx

@@ -226,11 +226,19 @@ int cmcx_set_params(cmcx_solver* s, const cmcx_params* params);
    Any array pointer may be NULL.  One grid, CPU merge order, axes of at most
    256 nodes, no BC_FREE node shared by two segments (else
    CMCX_ERR_UNSUPPORTED).  The current state is unchanged by the backward pass.
-   cmcx_adjoint_release frees the checkpoints and the recording. */
+   cmcx_adjoint_release frees the checkpoints and the recording.  The recording
+   belongs to the parameters, node values, BC kinds and options it was made
+   with: cmcx_set_params, cmcx_set_boundary and cmcx_set_options (with another
+   path or merge order) clear it, and cmcx_adjoint_backward then returns
+   CMCX_ERR_STATE.  cmcx_adjoint_generation gives the current recording's id
+   (it changes with every recording and every clearing; 0 = none), so that a
+   caller holding several rollouts can tell whether the one it wants a gradient
+   for is still the recorded one. */
 int cmcx_adjoint_forward(cmcx_solver* s, double dt, int steps, int num_global, int checkpoint_every);
 int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* gbar,
                           void* const* sbar, double* pbar);
 int cmcx_adjoint_release(cmcx_solver* s);
+int cmcx_adjoint_generation(cmcx_solver* s, uint64_t* generation);
 
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);

@@ -43,3 +43,40 @@ def probe_ke(fields, mask):
     for k, a in enumerate((u, v, w)):
         seed[k][mask] = a[mask] / n
     return loss, seed
+
+
+class ChannelProblem:
+    """The inlet-control gradient workload: a channel with a seeded z-axis
+    cylinder (domains.cylinder_channel), loss = mean kinetic energy over a
+    `probe`^3 box behind the cylinder (x in [nx/2+1, nx/2+1+probe), centred in
+    y and z), controls = the coarse inlet profile (each value fills a 3x3 block
+    of inlet nodes) and the kinematic viscosity nu = v_vis."""
+
+    def __init__(self, n: int = 96, steps: int = 200, probe: int = 16, **kw):
+        self.d, self.control = domains.cylinder_channel(n=n, steps=steps, **kw)
+        d = self.d
+        self.steps = steps
+        nx, ny, nz = d.shape
+        m = np.zeros(d.shape, bool)
+        x0 = nx // 2 + 1
+        m[x0:x0 + probe, ny // 2 - probe // 2:ny // 2 + probe // 2,
+          nz // 2 - probe // 2:nz // 2 + probe // 2] = True
+        self.mask = m.ravel()
+        self.ids = np.arange(ny * nz)  # inlet face x = 0, (y, z) order
+        self.nc = self.control.shape[0]
+
+    def inlet(self, c):
+        """Node u values with the inlet face set from control c."""
+        d = self.d
+        vx = d.vx.copy()
+        vx[self.ids] = np.kron(c, np.ones((3, 3)))[:d.ny, :d.nz].ravel()
+        return vx
+
+    def block_sum(self, g):
+        """Gradient of the node u values -> gradient of the control."""
+        d, nc = self.d, self.nc
+        g = np.asarray(g, np.float64)[self.ids].reshape(d.ny, d.nz)
+        return g[:3 * nc, :3 * nc].reshape(nc, 3, nc, 3).sum(axis=(1, 3))
+
+    def loss(self, fields):
+        return probe_ke(fields, self.mask)

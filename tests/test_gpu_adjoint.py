@@ -177,3 +177,39 @@ def test_adjoint_rejects_batch():
     with pytest.raises(Exception):
         s.adjoint_forward(2, 1e-3, 2, 1)
     s.close()
+
+
+def test_recording_belongs_to_its_rollout():
+    """A solver holds one recording: a second rollout recorded before the first
+    one's backward, or new parameters or node values, make the first backward
+    raise instead of returning another rollout's gradient."""
+    torch = pytest.importorskip("torch")
+    from cmcx._lib import CmcxError
+    from cmcx.autograd import channel_rollout
+
+    d, control = _channel(n=24, steps=4)
+    mask = torch.from_numpy(ac.probe_mask(d.shape, size=6, x0=2))
+    s = make_solver(d, "f64", path=FUSED)
+    say("[recording generation]")
+    c = torch.tensor(control, requires_grad=True)
+    nu = torch.tensor(d.params().v_vis, dtype=torch.float64, requires_grad=True)
+    f1 = channel_rollout(s, d, c, nu, d.steps, every=2)
+    g1 = s.adjoint_generation()
+    assert g1 != 0
+    f2 = channel_rollout(s, d, c * 1.01, nu, d.steps, every=2)
+    assert s.adjoint_generation() not in (0, g1)
+    with pytest.raises(RuntimeError, match="no longer this rollout"):
+        (f1[0][mask] ** 2).sum().backward()
+    (f2[0][mask] ** 2).sum().backward()  # the live recording still works
+    assert c.grad is not None and torch.isfinite(c.grad).all()
+    # parameter and node-value changes clear the recording at the C ABI too
+    s.adjoint_forward(2, d.dt, d.num_global, 1)
+    s.set_params(d.params())
+    assert s.adjoint_generation() == 0
+    with pytest.raises(CmcxError, match="set_params"):
+        s.adjoint_backward(None)
+    s.adjoint_forward(2, d.dt, d.num_global, 1)
+    s.set_boundary(d.bc_vel, d.bc_temp, d.vx, d.vy, d.vz, d.T)
+    with pytest.raises(CmcxError, match="set_boundary"):
+        s.adjoint_backward(None)
+    s.close()

@@ -235,3 +235,39 @@ def test_short_boundary_lists_bit_identical(dtype, path, graph):
     assert rel_l2(fa[4], want[3]) < tol
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("path", [PLAIN, FUSED])
+def test_blown_up_state_fires_guard(path):
+    """A NaN in the state (a run that blew up) fires the divergence guard: the
+    step returns CMCX_DIVERGED and leaves the state unadvanced.  The reference
+    tests err > 0.01, which a NaN passes; a grid without fluid nodes (0/0, see
+    test_no_fluid_node) still passes.  In a batch, a NaN in any grid fires it."""
+    from cmcx._lib import DivergenceError
+    from cmcx.solver import make_batch
+
+    d = domains.lid_sphere(24, radius=5.0, dt=1e-3)
+    say(f"[blown-up state] path={path}")
+    s = make_solver(d, "f32", path=path)
+    u, v, w, p, T = s.fields()
+    fluid = np.flatnonzero(d.type == domains.NODE_IN)
+    u[fluid[len(fluid) // 2]] = np.nan
+    s.set_fields(u, v, w, T)
+    with pytest.raises(DivergenceError):
+        s.TimeStep(d.dt, 2, 1, True)
+    got = s.fields()
+    assert np.isnan(got[0][fluid[len(fluid) // 2]])  # unadvanced: still the state set
+    assert np.array_equal(got[1], v)
+    s.close()
+    doms = domains.batch_boxes(count=3, seed=2, edges=(12, 16))
+    b = make_batch(doms, "f32", path=path)
+    b.select(2)
+    u, v, w, p, T = b.fields()
+    fl = np.flatnonzero(doms[2].type == domains.NODE_IN)
+    T[fl[0]] = np.inf
+    b.set_fields(u, v, w, T)
+    with pytest.raises(DivergenceError):
+        b.TimeStep(doms[0].dt, 2, 1, True)
+    errs = b.grid_errors()
+    assert np.all(np.isfinite(errs[:2])) and not np.isfinite(errs[2])
+    b.close()
