@@ -403,9 +403,27 @@ def measured_traffic(size: int, path: str):
     return None
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _nproc() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(dom, budget_s: float = 20.0):
     """Oracle (C++/OpenMP port of the reference CPU path) on a bounded sample:
-    whole FP32 time steps of the same workload until `budget_s` is spent."""
+    whole FP32 time steps of the same workload on all the OpenMP threads until
+    about `budget_s` is spent, then one step on one thread."""
     import oracle_lib
 
     L = oracle_lib.load()
@@ -421,13 +439,61 @@ def cpu_baseline(dom, budget_s: float = 20.0):
         if time.perf_counter() - t0 > budget_s or n >= 50:
             break
     wall = time.perf_counter() - t0
+    # one more step on one thread
+    L.orc_set_threads(1)
+    t1 = time.perf_counter()
+    o.update_boundaries()
+    o.step(dom.dt, dom.num_global, dom.num_local, False)
+    wall1 = time.perf_counter() - t1
+    L.orc_set_threads(threads)
     o.close()
     return {
         "value": dom.n * n / wall,
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n} full FP32 time step(s) of the same 256^3 workload (G=4, L=1), {wall:.1f} s, OpenMP",
+        "sample": f"{n} full FP32 time step(s) of the same 256^3 workload (G=4, L=1), {wall:.1f} s, "
+                  f"OpenMP on {threads} threads; then 1 step on 1 thread, {wall1:.1f} s",
+        "value_1thread": dom.n / wall1,
+        "nproc": _nproc(),
+        "cpu_model": _cpu_model(),
+    }
+
+
+def headline_parity(dom, variants, steps: int = 3):
+    """The headline path (FP32, the timed tile variants) against the FP64 oracle
+    after `steps` steps from the same initial state: relative L2 errors per
+    component, each relative to its own signal (u, v, w; T - baseT; p - v_T
+    baseT) and velocity-norm-scaled."""
+    import oracle_lib
+    from cmcx.solver import make_solver
+    from gpu_util import ROUNDOFF_FP32, component_errors
+
+    s = make_solver(dom, "f32", path=1)
+    for a, v in enumerate(variants):
+        s.set_tile_variant(a, v)
+    for _ in range(steps):
+        s.UpdateBoundaries()
+        s.TimeStep(dom.dt, dom.num_global, dom.num_local, False)
+    u, v, w, _, T = s.fields()
+    s.close()
+    o = oracle_lib.Oracle3D(dom, "f64")
+    o.create_segments()
+    for _ in range(steps):
+        o.update_boundaries()
+        o.step(dom.dt, dom.num_global, dom.num_local, False)
+    ref = o.fields()
+    o.close()
+    e = component_errors((u, v, w, T), ref, dom.base_t, dom.params().v_T, ROUNDOFF_FP32)
+    return {
+        "steps": steps,
+        "tile_variants": list(variants),
+        "own": {k: x[0] for k, x in e.items()},
+        "scaled": {k: x[1] for k, x in e.items()},
+        "max_rel_l2_vs_fp64_oracle": max(e[k][1] for k in ("u", "v", "w", "p")),
+        "note": "scaled: velocity components over the velocity-field norm, T and p over their own "
+                "norms (the north-star metric, bar 1e-5); own: each component over its own signal "
+                "(T - baseT, p - v_T baseT); None = zero up to FP32 rounding",
     }
 
 
@@ -443,6 +509,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-2d", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the inlet-control gradient workload and print its JSON")
     ap.add_argument("--adjoint-dirs", type=int, default=16,
@@ -501,6 +568,8 @@ def main():
     adjoint = None if args.no_adjoint else adjoint_workload(ndir=args.adjoint_dirs)
     two_d = None if args.no_2d else channel2d_workload()
     cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
+    parity = (None if args.no_parity or head_path != "fused"
+              else headline_parity(dom, head["variants"]))
 
     achieved = head["sweep_achieved_gbs"]
     line = {
@@ -567,6 +636,8 @@ def main():
         line["adjoint"] = adjoint
     if two_d is not None:
         line["channel2d"] = two_d
+    if parity is not None:
+        line["parity"] = parity
     if plain is not None:
         line["plain"] = {
             "value": plain["cell_updates_per_s"],

@@ -241,6 +241,9 @@ __device__ __forceinline__ void strided_chunk(const Geom& g, const SweepParams<T
         if (j == 0) rfirst = rw;
         else if (j == M - 1) rlast = rw;
         else chunk_down<T, M>(ch, j, rw);
+        // keep the pipeline's shape: the scheduler would otherwise hoist the
+        // loads of every row to the top and spill what does not fit
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (M >= 3) chunk_up<T, M>(ch);
     chunk_reduced<T, M>(ch, rfirst, rlast, f, l);
@@ -249,16 +252,49 @@ __device__ __forceinline__ void strided_chunk(const Geom& g, const SweepParams<T
 // Writes the chunk's rows given the corrections df / dl of its first and last
 // rows: x = t + delta on every row of a system (the node value on Dirichlet
 // rows), the merged Picard layer on NODE_IN rows.  Row j lives at element
-// base + (r0 + j) * s.
+// base + (r0 + j) * s.  The corrections of all rows are formed first (the
+// chunk state dies there), then the Picard layer of all rows is read, then
+// everything is stored: a load issued after a store waits for the store's
+// acknowledgement as well (one counter covers both on this architecture), so
+// reads interleaved with the stores would cost a memory round trip per field.
 template <class T, int M, bool KEEP, bool BSYNC>
 __device__ __forceinline__ void strided_write(const Chunk<T, M>& ch, const int* role,
                                               const uint8_t* __restrict__ type, int base, int s,
-                                              int r0, const T* df, const T* dl,
+                                              int safe, int r0, const T* df, const T* dl,
                                               const T* const* gval, const Layer4<T>& tin,
                                               const Layer4<T>& out, const Layer4<T>& tout,
                                               int merges, int stores, const T* keep, int kfs,
                                               int krs) {
+    // the row addresses are formed here, not kept live from the kernel's start
+    // (where the compiler would hoist them and spill them across the solve)
+    asm volatile("" : "+v"(base), "+v"(r0), "+v"(safe));
     const T km = merges > 1 ? (T)0.75 : (T)0.5;
+    T dd[4][M];
+#pragma unroll
+    for (int j = 0; j < M; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            if (j == 0) dd[v][j] = df[v];
+            else if (j == M - 1) dd[v][j] = dl[v];
+            else {
+                const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
+                const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
+                dd[v][j] = ch.E[v][j] + F * df[v] + G * dl[v];
+            }
+        }
+    T t0[4][M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        const int c = role[j] != RL_NONE ? base + (r0 + j) * s : safe;
+#pragma unroll
+        for (int v = 0; v < 4; v++) t0[v][j] = KEEP ? keep[v * kfs + j * krs] : tin.f[v][c];
+    }
+    // every read above completes here, before the first store (otherwise the
+    // compiler sinks each read into the branch of its row's stores)
+#pragma unroll
+    for (int j = 0; j < M; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) asm volatile("" ::"v"(t0[v][j]));
 #pragma unroll
     for (int j = 0; j < M; j++) {
         if (role[j] == RL_NONE) continue;
@@ -266,19 +302,10 @@ __device__ __forceinline__ void strided_write(const Chunk<T, M>& ch, const int* 
         const uint8_t tf = BSYNC ? (uint8_t)0 : type[c];
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            T d;
-            if (j == 0) d = df[v];
-            else if (j == M - 1) d = dl[v];
-            else {
-                const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
-                const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                d = ch.E[v][j] + F * df[v] + G * dl[v];
-            }
-            const T t0 = KEEP ? keep[v * kfs + j * krs] : tin.f[v][c];
-            T x = t0 + d;
+            T x = t0[v][j] + dd[v][j];
             if (!BSYNC && dirichlet_row(role[j], tf, v < 3 ? 0 : 1)) x = gval[v][c];
             if (stores & 1) out.f[v][c] = x;
-            if ((stores & 2) && role[j] == RL_IN) tout.f[v][c] = t0 + km * d;
+            if ((stores & 2) && role[j] == RL_IN) tout.f[v][c] = t0[v][j] + km * dd[v][j];
         }
     }
 }
@@ -397,8 +424,8 @@ __device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ ins
             xl[v] = red[L.at(gl, 6 + v, 2 * chunk + 1)];
         }
     }
-    strided_write<T, M, KEEP, BSYNC>(ch, role, type, base, s, r0, xf, xl, gval, tin, out, tout,
-                                     merges, stores, keep, kfs, krs);
+    strided_write<T, M, KEEP, BSYNC>(ch, role, type, base, s, safe, r0, xf, xl, gval, tin, out,
+                                     tout, merges, stores, keep, kfs, krs);
 }
 
 // ---------------------------------------------------------------------------
