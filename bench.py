@@ -70,6 +70,37 @@ def _sync():
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
 
 
+class DeviceFields:
+    """`k` device arrays of `n` elements (hipMalloc in the solver's runtime)."""
+
+    def __init__(self, k: int, n: int, dtype):
+        import ctypes
+
+        self.h = _hip()
+        self.n, self.dtype = n, np.dtype(dtype)
+        self.ptrs = []
+        for _ in range(k):
+            p = ctypes.c_void_p()
+            assert self.h.hipMalloc(ctypes.byref(p), n * self.dtype.itemsize) == 0
+            self.ptrs.append(p.value)
+
+    def upload(self, arrays):
+        for p, a in zip(self.ptrs, arrays):
+            a = np.ascontiguousarray(a, dtype=self.dtype)
+            assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return self.ptrs
+
+    def download(self, i):
+        out = np.empty(self.n, self.dtype)
+        assert self.h.hipMemcpy(out.ctypes.data, self.ptrs[i], out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.h.hipFree(p)
+        self.ptrs = []
+
+
 def copy_calibration(mib: int):
     """A plain device-to-device copy of `mib` MiB (3 times), the reference kernel
     for scaling the PMC byte counters."""
@@ -256,12 +287,26 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
         _sync()
         out["recorded_forward_s"] = time.perf_counter() - t0
         loss, seed = _probe_ke(s.fields(), mask)
+        # backward with the seed and the gradients in device memory (a caller
+        # whose loss lives on the GPU); the host-array call is timed beside it
+        dev = DeviceFields(4, d.n, s.np_dtype)
+        sd = dev.upload(seed)
+        gdev = DeviceFields(4, d.n, s.np_dtype)
+        _sync()
         t0 = time.perf_counter()
-        gb, _, pb = s.adjoint_backward(seed)
+        pb = s.adjoint_backward_device(sd, gdev.ptrs, None)
         _sync()
         out["backward_s"] = time.perf_counter() - t0
+        gb0 = gdev.download(0)
+        dev.free()
+        gdev.free()
+        t0 = time.perf_counter()
+        gbh, _, pbh = s.adjoint_backward(seed)
+        _sync()
+        out["backward_host_s"] = time.perf_counter() - t0
+        assert np.array_equal(gbh[0], gb0) and np.array_equal(pbh, pb)
         out["loss"] = loss
-        out["gc"] = block_sum(gb[0].astype(np.float64))
+        out["gc"] = block_sum(gb0.astype(np.float64))
         out["gnu"] = float(pb[1])
         s.adjoint_release()
         if timed:
@@ -327,13 +372,16 @@ def adjoint_workload(ndir: int = 16, steps: int = 200, every: int = 20, n: int =
         "forward_ms": r32["forward_s"] * 1e3,
         "recorded_forward_ms": r32["recorded_forward_s"] * 1e3,
         "backward_ms": r32["backward_s"] * 1e3,
+        "backward_host_arrays_ms": r32["backward_host_s"] * 1e3,
         "adjoint_forward_ratio": r32["backward_s"] / r32["forward_s"],
         "reverse_step_ms": r32["reverse_step_s"] * 1e3,
         "forward_step_ms": r32["forward_s"] / steps * 1e3,
         "adjoint_step_ratio": r32["reverse_step_s"] / (r32["forward_s"] / steps),
         "backward_call_fixed_ms": r32["reverse_fixed_s"] * 1e3,
-        "ratio_note": "adjoint_forward_ratio = checkpointed backward (every 20, recompute and "
-                      "host transfers included) / forward; adjoint_step_ratio = one reversed step "
+        "ratio_note": "adjoint_forward_ratio = checkpointed backward (every 20, every interval "
+                      "but the last recomputed; seed and gradients in device memory) / forward "
+                      "(200 steps, direct launches); backward_host_arrays_ms = the same backward "
+                      "taking and returning host arrays; adjoint_step_ratio = one reversed step "
                       "(all steps taped; difference of backwards over 200 and 100 steps) / one "
                       "forward step",
         "loss_fp32": r32["loss"],

@@ -89,6 +89,8 @@ SIGNATURES = [
     ("cmcx_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("cmcx_adjoint_backward", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                              ctypes.POINTER(_vp), _dp]),
+    ("cmcx_adjoint_backward_device", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                    ctypes.POINTER(_vp), _dp]),
     ("cmcx_adjoint_release", ctypes.c_int, [_vp]),
     ("cmcx_adjoint_generation", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     ("cmcx2d_create", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params), ctypes.c_int,

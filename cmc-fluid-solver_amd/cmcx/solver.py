@@ -232,6 +232,18 @@ class AdiSolver3D:
               "cmcx_adjoint_backward")
         return gb, sb, pb
 
+    def adjoint_backward_device(self, seed_ptrs, gbar_ptrs, sbar_ptrs):
+        """adjoint_backward with device buffers: each argument is 4 device
+        addresses (ints, None = absent) of N elements of the field type.
+        Returns pbar (4,)."""
+        def arr(ps):
+            return None if ps is None else (_lib._vp * 4)(*[p if p else None for p in ps])
+        pb = np.zeros(4, np.float64)
+        check(lib().cmcx_adjoint_backward_device(self.h, arr(seed_ptrs), arr(gbar_ptrs),
+                                                 arr(sbar_ptrs), pb.ctypes.data_as(_lib._dp)),
+              "cmcx_adjoint_backward_device")
+        return pb
+
     def adjoint_release(self):
         check(lib().cmcx_adjoint_release(self.h), "cmcx_adjoint_release")
 

@@ -63,7 +63,7 @@ struct ISolver {
     virtual int set_params(const cmcx_params* p) = 0;
     virtual int adjoint_forward(double dt, int steps, int G, int every) = 0;
     virtual int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
-                                 double* pbar) = 0;
+                                 double* pbar, bool device) = 0;
     virtual int adjoint_release() = 0;
     virtual int adjoint_generation(uint64_t* gen) = 0;
     virtual int synchronize() = 0;
@@ -156,7 +156,9 @@ class Solver final : public ISolver {
         T* tape = nullptr;     // per step: S, Picard layers 1..3G, sweep outputs 0..3G-1
         int64_t tape_layers = 0;
         int tape_G = 0;
-        bool last_taped = false;  // the tape holds the last interval of the recording
+        int nbuf = 0;              // tape buffers (two when the rollout has several intervals)
+        int holds[2] = {-1, -1};   // interval each tape buffer holds (-1 = none)
+        hipStream_t fs = nullptr;  // stream of the recompute, beside the reverse on `st`
         T* work = nullptr;     // xbar, tbar, state-adjoint sum, lam, node-value gradient, final state
         double* pp = nullptr;  // per-block parameter partials
         int32_t* tiles[3] = {nullptr, nullptr, nullptr};
@@ -323,6 +325,7 @@ class Solver final : public ISolver {
         hipFree(d_bvfree);
         hipFree(adj.ck);
         hipFree(adj.tape);
+        if (adj.fs) hipStreamDestroy(adj.fs);
         hipFree(adj.work);
         hipFree(adj.pp);
         for (auto* t : adj.tiles) hipFree(t);
@@ -422,7 +425,8 @@ class Solver final : public ISolver {
             gs.push_back(Gr.g);
             offs.push_back(Gr.off);
         }
-        build_fused_work(h_type.data(), gs, offs, a, strided_lpg<T>(v), sizeof(T), work[a][v]);
+        build_fused_work(h_type.data(), gs, offs, a, strided_lpg<T>(v), strided_chunk_rows<T>(v),
+                         sizeof(T), work[a][v]);
     }
 
     int build_segments() override {
@@ -1039,38 +1043,50 @@ class Solver final : public ISolver {
         // the intervals before it
         const int last0 = (nck - 1) * every;
         const int64_t per = 1 + 6 * (int64_t)G;
+        adj.holds[0] = adj.holds[1] = -1;
         for (int i = 0; i < steps; i++) {
             if (i % every == 0) copy_layer(lay(adj.ck, i / every), cur);
             if (i >= last0) {
-                record_step(adj.tape + (i - last0) * per * 4 * NT);
+                record_step(tape_buf((nck - 1) % adj.nbuf) + (i - last0) * per * 4 * NT);
             } else {
                 launch_step(true, dtt, G, 1);
                 std::swap(cur, next);
             }
         }
+        adj.holds[(nck - 1) % adj.nbuf] = nck - 1;
         adj.recorded = true;
         adj.generation++;
         adj.invalid_why.clear();
-        adj.last_taped = true;
         layers_ok = false;  // record_step leaves next and the Picard layers behind
         return CMCX_OK;
     }
 
-    // Tape of one checkpoint interval; the Picard-layer slots hold the node
+    // Tape buffer b (of adj.nbuf), each holding one checkpoint interval.
+    T* tape_buf(int b) const {
+        const int64_t per = 1 + 6 * (int64_t)adj.G;
+        return adj.tape + (int64_t)b * per * std::min(adj.every, adj.steps) * 4 * NT;
+    }
+
+    // Tape of the checkpoint intervals: two buffers when the rollout has more
+    // than one interval (the backward pass recomputes one interval while it
+    // reverses the next), else one.  The Picard-layer slots hold the node
     // values on non-IN nodes (the forward's temp holds the boundary-filled state
     // there, and the sweeps write NODE_IN nodes only).
     void ensure_tape(bool init_slots) {
         const int64_t per = 1 + 6 * (int64_t)adj.G;
         const int64_t nsteps_tape = std::min(adj.every, adj.steps);
-        const int64_t need = per * nsteps_tape;
+        const int nbuf = adj.steps > adj.every ? 2 : 1;
+        const int64_t need = per * nsteps_tape * nbuf;
         if (need > adj.tape_layers || adj.tape_G != adj.G) {
             hipFree(adj.tape);
             adj.tape = nullptr;
             CMCX_HIP(hipMalloc((void**)&adj.tape, sizeof(T) * 4 * NT * need));
             adj.tape_layers = need;
             adj.tape_G = adj.G;
+            adj.holds[0] = adj.holds[1] = -1;
             init_slots = true;
         }
+        adj.nbuf = nbuf;
         if (init_slots) {  // node values may have changed since the last recording
             Layer4<T> gl;
             for (int f = 0; f < 4; f++) gl.f[f] = d_g[f];
@@ -1130,7 +1146,7 @@ class Solver final : public ISolver {
     }
 
     int adjoint_backward(const void* const* seed, void* const* gbar, void* const* sbar,
-                         double* pbar) override {
+                         double* pbar, bool device) override {
         if (!adj.recorded) {
             set_error(adj.invalid_why.empty()
                           ? std::string("adjoint_backward before adjoint_forward")
@@ -1158,32 +1174,86 @@ class Solver final : public ISolver {
         if (seed)
             for (int f = 0; f < 4; f++)
                 if (seed[f])
-                    CMCX_HIP(hipMemcpyAsync(XB.f[f], seed[f], sizeof(T) * g.N, hipMemcpyHostToDevice, st));
+                    CMCX_HIP(hipMemcpyAsync(XB.f[f], seed[f], sizeof(T) * g.N,
+                                            device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        // Intervals are reversed last to first.  Interval i - 1 is recomputed
+        // (from its checkpoint, into the other tape buffer, on a stream of its
+        // own) while interval i is reversed on the solver's stream; events
+        // order a reverse after its interval's recompute, and a recompute
+        // after the reverse that last used its buffer.
         const int nseg = (adj.steps + adj.every - 1) / adj.every;
-        for (int sgm = nseg - 1; sgm >= 0; sgm--) {
-            const int s0 = sgm * adj.every;
-            const int k = std::min(adj.every, adj.steps - s0);
-            if (sgm != nseg - 1 || !adj.last_taped) {
+        const int nb = adj.nbuf;
+        if (!adj.fs) CMCX_HIP(hipStreamCreateWithFlags(&adj.fs, hipStreamNonBlocking));
+        std::vector<hipEvent_t> recd(nseg, nullptr), revd(nseg, nullptr);
+        hipEvent_t start = nullptr;
+        struct EvFree {
+            std::vector<hipEvent_t>* a;
+            std::vector<hipEvent_t>* b;
+            hipEvent_t* c;
+            ~EvFree() {
+                for (auto* v : {a, b})
+                    for (hipEvent_t e : *v)
+                        if (e) hipEventDestroy(e);
+                if (*c) hipEventDestroy(*c);
+            }
+        } evfree{&recd, &revd, &start};
+        for (int i = 0; i < nseg; i++) {
+            CMCX_HIP(hipEventCreateWithFlags(&recd[i], hipEventDisableTiming));
+            CMCX_HIP(hipEventCreateWithFlags(&revd[i], hipEventDisableTiming));
+        }
+        CMCX_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+        CMCX_HIP(hipEventRecord(start, st));
+        CMCX_HIP(hipStreamWaitEvent(adj.fs, start, 0));
+        hipStream_t main_st = st;
+        std::vector<bool> recomputed(nseg, false);
+        auto recompute = [&](int sgm) {
+            const int k = std::min(adj.every, adj.steps - sgm * adj.every);
+            const int b = sgm % nb;
+            if (sgm + 2 < nseg) CMCX_HIP(hipStreamWaitEvent(adj.fs, revd[sgm + 2], 0));
+            st = adj.fs;
+            try {
                 copy_layer(cur, lay(adj.ck, sgm));
                 bv_synced = false;
-                for (int i = 0; i < k; i++) record_step(adj.tape + i * per * 4 * NT);
+                for (int i = 0; i < k; i++) record_step(tape_buf(b) + i * per * 4 * NT);
+            } catch (...) {
+                st = main_st;
+                throw;
             }
-            for (int i = k - 1; i >= 0; i--) reverse_step(adj.tape + i * per * 4 * NT);
+            st = main_st;
+            CMCX_HIP(hipEventRecord(recd[sgm], adj.fs));
+            adj.holds[b] = sgm;
+            recomputed[sgm] = true;
+        };
+        if (adj.holds[(nseg - 1) % nb] != nseg - 1) recompute(nseg - 1);
+        for (int sgm = nseg - 1; sgm >= 0; sgm--) {
+            if (sgm > 0 && adj.holds[(sgm - 1) % nb] != sgm - 1) {
+                if (nb == 1) {
+                    // one buffer: the recompute waits for this interval's reverse
+                    // (single-interval rollouts never get here)
+                } else {
+                    recompute(sgm - 1);
+                }
+            }
+            const int k = std::min(adj.every, adj.steps - sgm * adj.every);
+            if (recomputed[sgm]) CMCX_HIP(hipStreamWaitEvent(st, recd[sgm], 0));
+            for (int i = k - 1; i >= 0; i--) reverse_step(tape_buf(sgm % nb) + i * per * 4 * NT);
+            CMCX_HIP(hipEventRecord(revd[sgm], st));
         }
         if (gbar)
             for (int f = 0; f < 4; f++)
                 if (gbar[f])
-                    CMCX_HIP(hipMemcpyAsync(gbar[f], GB.f[f], sizeof(T) * g.N, hipMemcpyDeviceToHost, st));
+                    CMCX_HIP(hipMemcpyAsync(gbar[f], GB.f[f], sizeof(T) * g.N,
+                                            device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         if (sbar)
             for (int f = 0; f < 4; f++)
                 if (sbar[f])
-                    CMCX_HIP(hipMemcpyAsync(sbar[f], XB.f[f], sizeof(T) * g.N, hipMemcpyDeviceToHost, st));
+                    CMCX_HIP(hipMemcpyAsync(sbar[f], XB.f[f], sizeof(T) * g.N,
+                                            device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         std::vector<double> pp(4 * adj_point_blocks());
         CMCX_HIP(hipMemcpyAsync(pp.data(), adj.pp, sizeof(double) * pp.size(), hipMemcpyDeviceToHost, st));
         copy_layer(cur, FIN);
         bv_synced = synced0;
         layers_ok = layers0;
-        adj.last_taped = nseg == 1;  // the tape now holds the first interval
         CMCX_HIP(hipStreamSynchronize(st));
         if (pbar) {
             for (int q = 0; q < 4; q++) pbar[q] = 0;
@@ -1217,8 +1287,8 @@ class Solver final : public ISolver {
         adj.tape = nullptr;
         adj.nck = 0;
         adj.tape_layers = 0;
+        adj.holds[0] = adj.holds[1] = -1;
         invalidate_recording("adjoint_release");
-        adj.last_taped = false;
         return CMCX_OK;
     }
 
@@ -1406,7 +1476,11 @@ int cmcx_adjoint_forward(cmcx_solver* s, double dt, int steps, int num_global, i
 }
 int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* gbar,
                           void* const* sbar, double* pbar) {
-    SOLVER_CALL(s, adjoint_backward(seed, gbar, sbar, pbar));
+    SOLVER_CALL(s, adjoint_backward(seed, gbar, sbar, pbar, false));
+}
+int cmcx_adjoint_backward_device(cmcx_solver* s, const void* const* seed, void* const* gbar,
+                                 void* const* sbar, double* pbar) {
+    SOLVER_CALL(s, adjoint_backward(seed, gbar, sbar, pbar, true));
 }
 int cmcx_adjoint_release(cmcx_solver* s) { SOLVER_CALL(s, adjoint_release()); }
 int cmcx_adjoint_generation(cmcx_solver* s, uint64_t* gen) { SOLVER_CALL(s, adjoint_generation(gen)); }

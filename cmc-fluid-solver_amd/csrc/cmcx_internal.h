@@ -106,10 +106,11 @@ struct InstDesc {
 // Work-group list of one fused sweep over all grids of a batch.
 struct FusedWork {
     int64_t nblocks = 0;
-    int32_t* d_items = nullptr;  // X/Y: {grid, tile} per work-group; Z: 2*kZWaves line ids
-                                 // (x*ny + y, -1 = padding) per work-group
+    int32_t* d_items = nullptr;  // X/Y: {grid, kmax tiles} per work-group; Z: 2*kZWaves
+                                 // slots (first line x*ny + y, line count) per work-group
     int32_t* d_inst = nullptr;   // Z: grid of each work-group
     int maxn = 0;                // longest line along the axis over the grids
+    int kmax = 1;                // X/Y: tiles per work item
     int vec_ok = 1;              // Z: every line starts on a 16-byte boundary
     void release();
 };
@@ -118,6 +119,8 @@ template <class T>
 bool fused_supported(const Geom& g);
 template <class T>
 int strided_lpg(int variant);
+template <class T>
+int strided_chunk_rows(int variant);
 // Strided variant v usable for lines of n nodes.
 template <class T>
 bool fused_variant_ok(int n, int v);
@@ -133,10 +136,12 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
                         const uint8_t* tflags, const T* const* gval, Layer4<T> in, Layer4<T> tin,
                         Layer4<T> out, Layer4<T> tout, int merges, int bsync, int stores,
                         int variant, hipStream_t st);
-// Builds the work of `axis` from the host copy of the flat node types; lpg: z
-// lines per tile of the strided (X, Y) sweeps; elem: sizeof the field type.
+// Builds the work of `axis` from the host copy of the flat node types; lpg and
+// m: z lines per tile and rows per chunk of the strided (X, Y) sweeps; elem:
+// sizeof the field type.  Tiles of grids with lines shorter than the longest
+// are packed several to a work-group (length binning inside one launch).
 void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
-                      const std::vector<int64_t>& offs, int axis, int lpg, size_t elem,
+                      const std::vector<int64_t>& offs, int axis, int lpg, int m, size_t elem,
                       FusedWork& w);
 // Axes whose lines hold a BC_FREE node with NODE_IN on both sides (a free node
 // shared by two segments): bit a set for axis a.  Host arrays of one grid.

@@ -332,7 +332,7 @@ __device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ ins
                                              const Layer4<T>& in, const Layer4<T>& tin,
                                              const Layer4<T>& out, const Layer4<T>& tout,
                                              int merges, int stores,
-                                             const int32_t* __restrict__ work) {
+                                             const int32_t* __restrict__ work, int kmax) {
     extern __shared__ char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);
     constexpr int GPW = 64 / LPG;  // chunks (lane groups) per wave
@@ -344,25 +344,35 @@ __device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ ins
     const int P = nw * GPW;
     const RedLayout L{2 * P, kRedFields * 2 * P + 1};
 
-    // work item: (instance, tile = outer * ztiles + z tile) of this work-group
+    // work item: the instance and up to kmax of its tiles (tile = outer *
+    // ztiles + z tile, -1 = none).  The work-group is sized for the longest
+    // line of the batch; a grid with shorter lines packs several tiles into
+    // it, tile k on chunks [k * cpl, (k + 1) * cpl) of each z-line slot.  The
+    // lines laid end to end form one block-diagonal system (a line starts and
+    // ends on a non-IN face node, whose row does not reach across), so the
+    // reduced solve treats them as one line.
     const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int inst = work[2 * b], tile = work[2 * b + 1];
+    const int32_t* item = work + (int64_t)b * (kmax + 1);
+    const int inst = item[0];
     const Geom g = insts[inst].g;
     SweepParams<T> p = insts[inst].p[AXIS];
     p.three_dt = three_dt;
     const int n = AXIS == AX_X ? g.nx : g.ny;
     const int s = AXIS == AX_X ? (int)g.sx : (int)g.sy;
+    const int cpl = (n + M - 1) / M;  // chunks per line of this grid
+    const int k = chunk / cpl;
+    const int tile = k < kmax ? item[1 + k] : -1;
     const int ztiles = (g.nz + LPG - 1) / LPG;
-    const int outer = tile / ztiles;
-    const int z = (tile % ztiles) * LPG + gl;
-    const bool zok = z < g.nz;
+    const int outer = tile >= 0 ? tile / ztiles : 0;
+    const int z = (tile >= 0 ? tile % ztiles : 0) * LPG + gl;
+    const bool zok = tile >= 0 && z < g.nz;
     // a grid of a batch holds fewer than 2^31 nodes with its offset (checked at
     // creation): 32-bit element indices
     const int off = (int)insts[inst].off;
     const int base = off + (AXIS == AX_X ? outer * (int)g.sy : outer * (int)g.sx) + z;
     // interior node (1, 1, 1): the element read by rows outside the line
     const int safe = off + (int)g.sx + (int)g.sy + 1;
-    const int r0 = chunk * M;
+    const int r0 = (chunk - k * cpl) * M;  // first row of the chunk in its line
 
     // KEEP: each lane parks the Picard layer of its rows in LDS, after the
     // reduced systems, [field][row][line], with a padded line pitch (the lane
@@ -373,7 +383,7 @@ __device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ ins
     const int red_elems = KEEP ? LPG * kNormFields * pow2_at_least(P) : LPG * L.pitch;
     const int krs = keep_row_pitch<T>(LPG, M);
     const int kfs = P * M * krs;
-    T* keep = red + red_elems + r0 * krs + gl;
+    T* keep = red + red_elems + chunk * M * krs + gl;
     int role[M];
     Chunk<T, M> ch;
     T f[kRedFields], l[kRedFields];
@@ -444,10 +454,10 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
     const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
     const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
-    int merges, int stores, const int32_t* __restrict__ work) {
+    int merges, int stores, const int32_t* __restrict__ work, int kmax) {
     const T* gval[4] = {gv0, gv1, gv2, gv3};
     strided_tile<T, AXIS, M, LPG, KEEP, BSYNC>(insts, three_dt, type, gval, in, tin, out, tout,
-                                               merges, stores, work);
+                                               merges, stores, work, kmax);
 }
 
 // Contiguous lines (Z sweep): 32 lanes per line, two lines per wavefront.  A
@@ -477,13 +487,19 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     const Geom g = insts[inst].g;
     SweepParams<T> p = insts[inst].p[AX_Z];
     p.three_dt = three_dt;
-    const int n = g.nz;
-    const int used = (n + M - 1) / M;  // lanes of a line holding rows (<= LPL)
 
-    // 2 * kZWaves lines per work-group, 64 / LPL per wavefront
+    // 2 * kZWaves slots per work-group, 64 / LPL per wavefront.  A slot holds a
+    // run of `cnt` consecutive z lines starting at line x*ny + y (-1 = none):
+    // adjacent in memory, they form one block-diagonal system (every line
+    // starts and ends on a non-IN face node), solved as one line of cnt * nz
+    // rows -- several short lines of a batch share the lanes sized for the
+    // longest.
     const int sub = lane / LPL, ll = lane % LPL;
-    const int lid = lines[b * (2 * kZWaves) + (64 / LPL) * wave + sub];  // x*ny + y, or -1
+    const int slot = b * (2 * kZWaves) + (64 / LPL) * wave + sub;
+    const int lid = lines[2 * slot], cnt = lines[2 * slot + 1];
     const bool live = lid >= 0;
+    const int n = live ? cnt * g.nz : 0;
+    const int used = (n + M - 1) / M;  // lanes of the slot holding rows (<= LPL)
     const int64_t base = live ? insts[inst].off + (int64_t)lid * g.nz : 0;
     const int r0 = ll * M;
     const bool ok = live && ll < used;
@@ -684,6 +700,11 @@ int strided_lpg(int variant) {
     return variant_lpg<T>(variant);
 }
 
+template <class T>
+int strided_chunk_rows(int variant) {
+    return variant_chunk<T>(variant);
+}
+
 constexpr size_t kMaxLds = 160 * 1024;
 
 template <class T>
@@ -775,7 +796,8 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
                                              32 * (kShapes[V].bytes / (int)sizeof(T)),            \
                                              16 / (int)sizeof(T), kShapes[V].keep, BS>),          \
                        dim3(blocks), dim3(64 * waves), lds, st, insts, three_dt, tflags, gval[0], \
-                       gval[1], gval[2], gval[3], in, tin, out, tout, merges, stores, w.d_items)
+                       gval[1], gval[2], gval[3], in, tin, out, tout, merges, stores, w.d_items, \
+                       w.kmax)
 #define LSA(AX)                                 \
     if (bsync) {                                \
         switch (variant) {                      \
@@ -822,38 +844,78 @@ void FusedWork::release() {
     nblocks = 0;
 }
 
-// Work of one axis over all instances.  Z: lines x*ny + y holding a NODE_IN
-// node, each instance's list padded with -1 to whole work-groups (2 lines per
-// wave), plus the instance of every work-group.  X/Y: (instance, tile) pairs,
-// tile = outer * ztiles + z tile, for the tiles of lpg z lines holding a
-// NODE_IN node.
+// Work of one axis over all instances.  The work-group of a launch is sized
+// for the longest line of the batch (`maxn`); grids with shorter lines pack
+// several tiles (X/Y) or line runs (Z) into one work-group, so that short
+// lines do not run in lanes sized for long ones.
+//   Z: slots of (first line x*ny + y, count): runs of consecutive lines
+//      holding a NODE_IN node, at most floor(capacity / nz) lines each
+//      (capacity = 32 lanes x the rows per lane of the longest line), each
+//      instance's slots padded with (-1, 0) to whole work-groups (2 per wave),
+//      plus the instance of every work-group.
+//   X/Y: items of kmax + 1 ints: (instance, tile, tile, ...), tile = outer *
+//      ztiles + z tile of the tiles of lpg z lines holding a NODE_IN node,
+//      -1-padded; an instance takes floor(P / chunks per line) tiles per item
+//      (P = chunks per z-line slot of the work-group, chunks of m rows).
 void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
-                      const std::vector<int64_t>& offs, int axis, int lpg, size_t elem,
+                      const std::vector<int64_t>& offs, int axis, int lpg, int m, size_t elem,
                       FusedWork& w) {
     w.release();
     std::vector<int32_t> items, winst;
     w.maxn = 0;
     w.vec_ok = 1;
-    for (size_t i = 0; i < geoms.size(); i++) {
-        const Geom& g = geoms[i];
-        const uint8_t* t = h_type + offs[i];
-        if (axis == AX_Z) {
-            w.maxn = std::max(w.maxn, g.nz);
+    w.kmax = 1;
+    for (const Geom& g : geoms)
+        w.maxn = std::max(w.maxn, axis == AX_Z ? g.nz : axis == AX_X ? g.nx : g.ny);
+    if (axis == AX_Z) {
+        const int cap = 32 * std::max(z_chunk(w.maxn, 32), 2);
+        for (size_t i = 0; i < geoms.size(); i++) {
+            const Geom& g = geoms[i];
+            const uint8_t* t = h_type + offs[i];
             if (((size_t)g.nz * elem) % 16 || ((size_t)offs[i] * elem) % 16) w.vec_ok = 0;
-            size_t first = items.size();
+            const int K = std::max(1, cap / g.nz);
+            const size_t first = items.size();
+            int run0 = -1, cnt = 0;
             for (int x = 0; x < g.nx; x++)
                 for (int y = 0; y < g.ny; y++) {
                     const uint8_t* row = &t[x * g.sx + y * g.sy];
                     bool any = false;
                     for (int z = 0; z < g.nz && !any; z++) any = row[z] == NODE_IN;
-                    if (any) items.push_back(x * g.ny + y);
+                    if (!any) continue;
+                    const int lid = x * g.ny + y;
+                    if (cnt > 0 && lid == run0 + cnt && cnt < K) {
+                        cnt++;
+                        continue;
+                    }
+                    if (cnt > 0) {
+                        items.push_back(run0);
+                        items.push_back(cnt);
+                    }
+                    run0 = lid;
+                    cnt = 1;
                 }
-            const size_t per = 2 * kZWaves;
-            while ((items.size() - first) % per) items.push_back(-1);
+            if (cnt > 0) {
+                items.push_back(run0);
+                items.push_back(cnt);
+            }
+            const size_t per = 2 * 2 * kZWaves;
+            while ((items.size() - first) % per) {
+                items.push_back(-1);
+                items.push_back(0);
+            }
             for (size_t k = first; k < items.size(); k += per) winst.push_back((int32_t)i);
-        } else {
+        }
+        w.nblocks = (int64_t)winst.size();
+    } else {
+        const int P = strided_waves_for(w.maxn, lpg, m) * (64 / lpg);
+        std::vector<std::vector<int32_t>> tiles(geoms.size());
+        std::vector<int> per(geoms.size());
+        for (size_t i = 0; i < geoms.size(); i++) {
+            const Geom& g = geoms[i];
+            const uint8_t* t = h_type + offs[i];
             const int n = axis == AX_X ? g.nx : g.ny;
-            w.maxn = std::max(w.maxn, n);
+            per[i] = std::max(1, P / ((n + m - 1) / m));
+            w.kmax = std::max(w.kmax, per[i]);
             const int ztiles = (g.nz + lpg - 1) / lpg;
             const int nouter = axis == AX_X ? g.ny : g.nx;
             const int64_t so = axis == AX_X ? g.sy : g.sx, sr = axis == AX_X ? g.sx : g.sy;
@@ -863,24 +925,26 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
                     for (int r = 0; r < n && !any; r++)
                         for (int z = zt * lpg; z < std::min(g.nz, (zt + 1) * lpg) && !any; z++)
                             any = t[o * so + r * sr + z] == NODE_IN;
-                    if (any) {
-                        items.push_back((int32_t)i);
-                        items.push_back(o * ztiles + zt);
-                    }
+                    if (any) tiles[i].push_back(o * ztiles + zt);
                 }
         }
-    }
-    if (axis == AX_Y) {
-        // The Y sweep reads what the Z sweep wrote: x-planes, in list order
-        // per XCD.  Walking its own list backwards, it starts on the planes Z
-        // wrote last, which the memory-side cache still holds.
-        const size_t np = items.size() / 2;
-        for (size_t k = 0; k < np / 2; k++) {
-            std::swap(items[2 * k], items[2 * (np - 1 - k)]);
-            std::swap(items[2 * k + 1], items[2 * (np - 1 - k) + 1]);
+        const int W = w.kmax + 1;
+        for (size_t i = 0; i < geoms.size(); i++)
+            for (size_t k = 0; k < tiles[i].size(); k += per[i]) {
+                items.push_back((int32_t)i);
+                for (int q = 0; q < w.kmax; q++)
+                    items.push_back(q < per[i] && k + q < tiles[i].size() ? tiles[i][k + q] : -1);
+            }
+        w.nblocks = (int64_t)items.size() / W;
+        if (axis == AX_Y) {
+            // The Y sweep reads what the Z sweep wrote: x-planes, in list order
+            // per XCD.  Walking its own list backwards, it starts on the planes Z
+            // wrote last, which the memory-side cache still holds.
+            const int64_t nb = w.nblocks;
+            for (int64_t k = 0; k < nb / 2; k++)
+                for (int q = 0; q < W; q++) std::swap(items[k * W + q], items[(nb - 1 - k) * W + q]);
         }
     }
-    w.nblocks = axis == AX_Z ? (int64_t)winst.size() : (int64_t)items.size() / 2;
     if (!items.empty()) {
         CMCX_HIP(hipMalloc((void**)&w.d_items, sizeof(int32_t) * items.size()));
         CMCX_HIP(hipMemcpy(w.d_items, items.data(), sizeof(int32_t) * items.size(),
@@ -925,6 +989,8 @@ template bool fused_variant_ok<float>(int, int);
 template bool fused_variant_ok<double>(int, int);
 template int strided_lpg<float>(int);
 template int strided_lpg<double>(int);
+template int strided_chunk_rows<float>(int);
+template int strided_chunk_rows<double>(int);
 template void launch_fused_sweep<float>(const InstDesc<float>*, float, int, const FusedWork&,
                                         const uint8_t*, const float* const*, Layer4<float>,
                                         Layer4<float>, Layer4<float>, Layer4<float>, int, int, int,

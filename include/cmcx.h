@@ -222,7 +222,9 @@ int cmcx_set_params(cmcx_solver* s, const cmcx_params* params);
    sbar[4] = dL/d(initial state) on NODE_IN nodes and pbar[4] = dL/d(v_T, v_vis,
    t_vis, t_phi).  Each checkpoint interval is recomputed once with every
    sweep's layers kept on the device, then reversed step by step (transposed
-   line solves in reverse sweep order and the adjoint of the Picard merges).
+   line solves in reverse sweep order and the adjoint of the Picard merges);
+   an interval's recompute runs on a second stream while the interval after it
+   is reversed (two tape buffers).
    Any array pointer may be NULL.  One grid, CPU merge order, axes of at most
    256 nodes, no BC_FREE node shared by two segments (else
    CMCX_ERR_UNSUPPORTED).  The current state is unchanged by the backward pass.
@@ -237,6 +239,10 @@ int cmcx_set_params(cmcx_solver* s, const cmcx_params* params);
 int cmcx_adjoint_forward(cmcx_solver* s, double dt, int steps, int num_global, int checkpoint_every);
 int cmcx_adjoint_backward(cmcx_solver* s, const void* const* seed, void* const* gbar,
                           void* const* sbar, double* pbar);
+/* The same with seed, gbar and sbar in device memory (N elements of the field
+   type each, on the handle's device): no host transfer of the fields. */
+int cmcx_adjoint_backward_device(cmcx_solver* s, const void* const* seed, void* const* gbar,
+                                 void* const* sbar, double* pbar);
 int cmcx_adjoint_release(cmcx_solver* s);
 int cmcx_adjoint_generation(cmcx_solver* s, uint64_t* generation);
 

@@ -213,3 +213,46 @@ def test_recording_belongs_to_its_rollout():
     with pytest.raises(CmcxError, match="set_boundary"):
         s.adjoint_backward(None)
     s.close()
+
+
+def test_backward_device_buffers_match_host():
+    """cmcx_adjoint_backward_device (seed and gradients in device memory) gives
+    the host call's gradients bit for bit, over a rollout of several checkpoint
+    intervals (recompute and reverse overlapped on two streams)."""
+    import ctypes
+
+    from cmcx._lib import lib
+
+    lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    d, control = _channel(n=24, steps=7)
+    mask = ac.probe_mask(d.shape, size=6, x0=2)
+    say("[adjoint device buffers]")
+    for dtype in ("f32", "f64"):
+        s = make_solver(d, dtype, path=FUSED)
+        s.adjoint_forward(d.steps, d.dt, d.num_global, 2)  # intervals 2, 2, 2, 1
+        _, seed = ac.probe_ke(s.fields(), mask)
+        gh, sh, ph = s.adjoint_backward(seed)
+        nbytes = d.n * np.dtype(s.np_dtype).itemsize
+        bufs = []
+        for _ in range(12):
+            p = ctypes.c_void_p()
+            assert hip.hipMalloc(ctypes.byref(p), nbytes) == 0
+            bufs.append(p.value)
+        for f in range(4):
+            a = np.ascontiguousarray(seed[f], dtype=s.np_dtype)
+            assert hip.hipMemcpy(bufs[f], a.ctypes.data, nbytes, 1) == 0
+        pd = s.adjoint_backward_device(bufs[0:4], bufs[4:8], bufs[8:12])
+        for f in range(4):
+            g = np.empty(d.n, s.np_dtype)
+            assert hip.hipMemcpy(g.ctypes.data, bufs[4 + f], nbytes, 2) == 0
+            assert np.array_equal(g, gh[f]), (dtype, f)
+            assert hip.hipMemcpy(g.ctypes.data, bufs[8 + f], nbytes, 2) == 0
+            assert np.array_equal(g, sh[f]), (dtype, f)
+        assert np.array_equal(pd, ph)
+        for p in bufs:
+            hip.hipFree(p)
+        s.close()
