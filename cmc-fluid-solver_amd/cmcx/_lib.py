@@ -76,6 +76,7 @@ SIGNATURES = [
     ("cmcx_profile", ctypes.c_int, [_vp, ctypes.c_int]),
     ("cmcx_profile_read", ctypes.c_int, [_vp, _dp, _i64p, ctypes.c_int]),
     ("cmcx_step_bytes", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    ("cmcx_div_error_fields", ctypes.c_int, [_vp, _vp, _vp, _vp, _i32p, _dp, ctypes.c_int, _vp, _dp]),
     ("cmcx_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("cmcx_set_tile_variant", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     ("cmcx_synchronize", ctypes.c_int, [_vp]),

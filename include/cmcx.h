@@ -246,6 +246,21 @@ int cmcx_adjoint_backward_device(cmcx_solver* s, const void* const* seed, void* 
 int cmcx_adjoint_release(cmcx_solver* s);
 int cmcx_adjoint_generation(cmcx_solver* s, uint64_t* generation);
 
+/* EvalDivError of fields already in device memory, allocating nothing: the
+   variant of cmcx_div_error for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops).  No counterpart in the
+   reference.  u, v, w: device arrays of the field type (dtype), type: device
+   array of node types, x-major with z fastest, dims[3] each >= 3 with fewer
+   than 2^31 nodes in all (else CMCX_ERR_TOO_LARGE), h[3] the spacings.  The
+   reduction's partial sums live in a fixed device array of the library
+   (2 + 4096 doubles, held by each GPU the library runs on until the process
+   ends), whatever the grid size; calls that overlap on one GPU race on it.
+   Work is ordered on `stream` (NULL = default stream); *err is written after
+   the stream has finished the call's work. */
+int cmcx_div_error_fields(const void* u, const void* v, const void* w, const uint8_t* type,
+                          const int32_t* dims, const double* h, int dtype, void* stream,
+                          double* err);
+
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
 int cmcx_synchronize(cmcx_solver* s);
