@@ -2,8 +2,8 @@
 
     python cmc-fluid-solver_amd/build.py          # -> cmc-fluid-solver_amd/lib/libcmcx.so
 
-Each csrc/*.hip is compiled by hipcc for gfx950 in parallel, then linked into
-one shared library.  Packed FP32 instructions are disabled for every device
+Each csrc/*.hip (and the host-only csrc/*.cpp) is compiled by hipcc for gfx950
+in parallel, then linked into one shared library.  Packed FP32 instructions are disabled for every device
 compile (-target-feature -packed-fp32-ops).
 """
 from __future__ import annotations
@@ -42,7 +42,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
-    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
     jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))

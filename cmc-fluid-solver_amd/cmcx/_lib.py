@@ -43,6 +43,13 @@ class cmcx2d_domain(ctypes.Structure):
                 ("type", _u8p), ("bc", _u8p), ("v", _dp * 2), ("T", _dp)]
 
 
+class cmcx_coards_header(ctypes.Structure):
+    _fields_ = [("bbox_min", ctypes.c_double * 3), ("bbox_max", ctypes.c_double * 3),
+                ("timestep", ctypes.c_double), ("time", ctypes.c_double),
+                ("outdims", ctypes.c_int32 * 3), ("vars", ctypes.c_char_p),
+                ("xy_degree_units", ctypes.c_int), ("depth", ctypes.POINTER(ctypes.c_float))]
+
+
 class cmcx_params(ctypes.Structure):
     _fields_ = [("v_T", ctypes.c_double), ("v_vis", ctypes.c_double), ("t_vis", ctypes.c_double),
                 ("t_phi", ctypes.c_double)]
@@ -94,6 +101,9 @@ SIGNATURES = [
                                                     ctypes.POINTER(_vp), _dp]),
     ("cmcx_adjoint_release", ctypes.c_int, [_vp]),
     ("cmcx_adjoint_generation", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    ("cmcx_coards_write_header", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cmcx_coards_header)]),
+    ("cmcx_coards_write_layer", ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int, _dp, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
     ("cmcx2d_create", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params), ctypes.c_int,
                                      _vp, ctypes.POINTER(_vp)]),
     ("cmcx2d_update_boundaries", ctypes.c_int, [_vp]),

@@ -24,6 +24,8 @@
  *   cmcx_set_boundary     Grid3D::SetData                   src/FluidSolver3D/Grid3D.h:121
  *   cmcx_params_*         FluidParams constructors          src/Common/Geometry.h:545-561
  *   cmcx_destroy          AdiSolver3D::~AdiSolver3D         src/FluidSolver3D/AdiSolver3D.cpp:153-158
+ *   cmcx_coards_write_header  OutputNetCDF3D_header        src/Common/IO.h:136-297
+ *   cmcx_coards_write_layer   OutputNetCDF3D_layer         src/Common/IO.h:355-388
  *
  * 2D ADI (the reference's CPU-only FluidSolver2D path):
  *   cmcx2d_create/update_boundaries/step/get_fields/destroy
@@ -264,6 +266,29 @@ int cmcx_div_error_fields(const void* u, const void* v, const void* w, const uin
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
 int cmcx_synchronize(cmcx_solver* s);
+
+/* ---- COARDS result files (the reference's netCDF output, IO.h:136-388) -----
+   Written as netCDF classic with 64-bit offsets (no netCDF library needed),
+   with the reference's data model: dimensions x, y, z, t (unlimited); float
+   x(x), y(y), z(z), double time(t); the requested variables among u, v, w, T
+   as double (t, x, y, z) and d as float (x, y); missing_value 99999 and the
+   reference's attributes.  The header announces time/timestep records
+   (time = i * timestep), their field slices holding netCDF's fill value until a
+   layer is written; writing a layer past the last record appends records. */
+typedef struct cmcx_coards_header {
+    double bbox_min[3], bbox_max[3];  /* physical extent (x and y axis values: min + i * (max-min)/n) */
+    double timestep;                  /* output interval (dt * out_time_steps) */
+    double time;                      /* final time */
+    int32_t outdims[3];               /* output grid (GetLayer's ox, oy, oz) */
+    const char* vars;                 /* output variables, e.g. "u v w T" (d = depth) */
+    int xy_degree_units;              /* x, y in degrees (bathymetry input) instead of metres */
+    const float* depth;               /* d at the output resolution, x-major (outdims[0]*outdims[1]), or NULL */
+} cmcx_coards_header;
+int cmcx_coards_write_header(const char* path, const cmcx_coards_header* header);
+/* One output layer (GetLayer's arrays: 3 velocity values per node of the field
+   type vel_dtype, double T) at record time_index. */
+int cmcx_coards_write_layer(const char* path, const void* vel, int vel_dtype, const double* T,
+                            int time_index, int dimx, int dimy, int dimz, const char* vars);
 
 /* ---- 2D ADI (the reference's FluidSolver2D path) -------------------------- */
 typedef struct cmcx2d_solver cmcx2d_solver;
