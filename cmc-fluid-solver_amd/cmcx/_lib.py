@@ -25,6 +25,7 @@ CMCX_ERR_PICARD = -7
 CMCX_F32, CMCX_F64 = 0, 1
 CMCX_PATH_PLAIN, CMCX_PATH_FUSED = 0, 1
 CMCX_MERGE_CPU, CMCX_MERGE_GPU_ORDER = 0, 1
+CMCX_SHAPE3D, CMCX_SHAPE2D = 3, 2
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -101,6 +102,11 @@ SIGNATURES = [
                                                     ctypes.POINTER(_vp), _dp]),
     ("cmcx_adjoint_release", ctypes.c_int, [_vp]),
     ("cmcx_adjoint_generation", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    ("cmcx_geometry_load", ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("cmcx_geometry_info", ctypes.c_int, [_vp, _i32p, _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp]),
+    ("cmcx_geometry_build", ctypes.c_int, [_vp, ctypes.c_double, _u8p, _u8p, _u8p, _dp, _dp, _dp, _dp]),
+    ("cmcx_geometry_destroy", ctypes.c_int, [_vp]),
     ("cmcx_coards_write_header", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cmcx_coards_header)]),
     ("cmcx_coards_write_layer", ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int, _dp, ctypes.c_int,
                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),

@@ -137,6 +137,18 @@ class Solver final : public ISolver {
     // layer, itself the state's); a boundary fill out of sync, a step out of
     // sync, new BC kinds or a state handed out by pointer clear it.
     bool layers_ok = true;
+    // GetLayer's NODE_OUT fill (Solver3D.cpp:21-25) writes 99999 into `next`,
+    // the buffer that holds the previous state, and there it stays: the
+    // reference's sweeps never write NODE_OUT nodes, so each of its two state
+    // buffers keeps its own NODE_OUT values.  The fused Z sweep stores whole
+    // runs and carries its input's NODE_OUT values into its output, which is
+    // the same thing while both buffers agree.  Per grid, bit b: state buffer b
+    // (b = 0 at buf, 1 after it) holds the fill.  A step between buffers that
+    // disagree keeps the output buffer's own values (saved in out_keep) or
+    // writes the fill again.
+    std::vector<uint8_t> out_fill;
+    T* keep_buf = nullptr;
+    Layer4<T> out_keep{};
     // HIP graphs of whole steps, per (boundary fill, dt, G, L, state buffer)
     bool graph_mode = false;
     // (with the boundary-sync mode the sweeps were captured with)
@@ -175,6 +187,7 @@ class Solver final : public ISolver {
     Solver(const cmcx_domain* doms, const cmcx_params* ps, int count, hipStream_t stream) {
         st = stream;
         grids.resize(count);
+        out_fill.assign(count, 0);
         int64_t off = 0;
         for (int i = 0; i < count; i++) {
             const cmcx_domain* d = &doms[i];
@@ -312,6 +325,7 @@ class Solver final : public ISolver {
         hipFree(d_tflags);
         for (auto* p : d_g) hipFree(p);
         hipFree(buf);
+        hipFree(keep_buf);
         for (auto& G : grids)
             for (auto& s : G.segs) hipFree(s.d);
         hipFree(scratch);
@@ -693,6 +707,39 @@ class Solver final : public ISolver {
         layers_ok = merge_order == CMCX_MERGE_CPU && bv_synced;
     }
 
+    int state_buffer(const Layer4<T>& L) const { return L.f[0] == buf ? 0 : 1; }
+
+    // launch_step, with each state buffer keeping its own NODE_OUT values
+    // (see out_fill).
+    void step_keeping_out_fill(bool ub, T dt, int G, int L) {
+        const int pc = state_buffer(cur), pn = state_buffer(next);
+        std::vector<int> differ;
+        for (size_t i = 0; i < grids.size(); i++)
+            if (((out_fill[i] >> pc) ^ (out_fill[i] >> pn)) & 1) differ.push_back((int)i);
+        for (int i : differ) {
+            const Grid& Gr = grids[i];
+            if ((out_fill[i] >> pn) & 1) continue;
+            if (!keep_buf) {
+                CMCX_HIP(hipMalloc((void**)&keep_buf, sizeof(T) * NT * 4));
+                for (int f = 0; f < 4; f++) out_keep.f[f] = keep_buf + (int64_t)f * NT;
+            }
+            launch_copy_typed<T>(Gr.g, d_type + Gr.off, NODE_OUT, shifted(next, Gr.off),
+                                 shifted(out_keep, Gr.off), st);
+        }
+        launch_step(ub, dt, G, L);
+        for (int i : differ) {
+            const Grid& Gr = grids[i];
+            if ((out_fill[i] >> pn) & 1) {
+                const T miss = (T)99999.0f;
+                launch_fill_typed<T>(Gr.g, d_type + Gr.off, NODE_OUT, shifted(next, Gr.off), miss,
+                                     miss, miss, miss, st);
+            } else {
+                launch_copy_typed<T>(Gr.g, d_type + Gr.off, NODE_OUT, shifted(out_keep, Gr.off),
+                                     shifted(next, Gr.off), st);
+            }
+        }
+    }
+
     // Launches the step (graph or direct), then the divergence guard and swap.
     int advance(bool ub, double dt_d, int G, int L, int compute_err, double* err) {
         if (!built) {
@@ -701,7 +748,7 @@ class Solver final : public ISolver {
         }
         if (G < 0 || L < 1) return CMCX_ERR_ARG;
         const T dt = (T)dt_d;
-        launch_step(ub, dt, G, L);
+        step_keeping_out_fill(ub, dt, G, L);
         if (compute_err) diff_error = eval_div(next);
         if (err) *err = diff_error;
         if (prof) harvest_events();
@@ -802,6 +849,7 @@ class Solver final : public ISolver {
                 CMCX_HIP(hipMemcpyAsync(cur.f[f] + Gr.off, src[f], sizeof(T) * Gr.g.N,
                                         hipMemcpyHostToDevice, st));
         copy_state_to_all(Gr.off, Gr.g.N);
+        out_fill[sel] = 0;
         bv_synced = false;
         CMCX_HIP(hipStreamSynchronize(st));
         return CMCX_OK;
@@ -825,6 +873,7 @@ class Solver final : public ISolver {
         // Solver3D::GetLayer: next->Clear(grid, NODE_OUT, 99999, ...) then FilterToArrays
         T miss = (T)99999.0f;
         launch_fill_typed<T>(g, d_type + Gr.off, NODE_OUT, shifted(next, Gr.off), miss, miss, miss, miss, st);
+        out_fill[sel] |= (uint8_t)(1 << state_buffer(next));
         const int64_t N = g.N;
         std::vector<T> h[4];
         for (int f = 0; f < 4; f++) {
@@ -1049,7 +1098,7 @@ class Solver final : public ISolver {
             if (i >= last0) {
                 record_step(tape_buf((nck - 1) % adj.nbuf) + (i - last0) * per * 4 * NT);
             } else {
-                launch_step(true, dtt, G, 1);
+                step_keeping_out_fill(true, dtt, G, 1);
                 std::swap(cur, next);
             }
         }

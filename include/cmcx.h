@@ -24,6 +24,9 @@
  *   cmcx_set_boundary     Grid3D::SetData                   src/FluidSolver3D/Grid3D.h:121
  *   cmcx_params_*         FluidParams constructors          src/Common/Geometry.h:545-561
  *   cmcx_destroy          AdiSolver3D::~AdiSolver3D         src/FluidSolver3D/AdiSolver3D.cpp:153-158
+ *   cmcx_geometry_load    Grid3D::LoadFromFile (Load3DShape, Grid2D::LoadFromFile)
+ *                         src/FluidSolver3D/Grid3D.cpp:373-436, 488-514; src/FluidSolver2D/Grid2D.cpp:272-356
+ *   cmcx_geometry_build   Grid3D::Prepare (Build, Prepare2D) src/FluidSolver3D/Grid3D.cpp:574-666, 864-981
  *   cmcx_coards_write_header  OutputNetCDF3D_header        src/Common/IO.h:136-297
  *   cmcx_coards_write_layer   OutputNetCDF3D_layer         src/Common/IO.h:355-388
  *
@@ -266,6 +269,31 @@ int cmcx_div_error_fields(const void* u, const void* v, const void* w, const uin
 /* Algorithmic HBM bytes of one time step: G*L*(16*w+1)*(S_X+S_Y+S_Z). */
 int cmcx_step_bytes(cmcx_solver* s, int num_global, int num_local, double* bytes);
 int cmcx_synchronize(cmcx_solver* s);
+
+/* ---- geometry ingestion (the reference's Shape3D / Shape2D input formats) ----
+   A triangle-mesh file (in_fmt Shape3D: frames of vertices with velocities,
+   then triangles) or a polyline file (in_fmt Shape2D: frames of polylines,
+   each "Passive" or "Motion" with a velocity, extruded `depth` deep with the
+   bottom raised by `depth_var`) becomes the node arrays of a 3D grid on the
+   GPU: rasterised walls, flood-filled outside, fluid inside, the reference's
+   BC kinds and node values.  Coordinates in the file are millimetres; spacing
+   and depth in metres; align rounds every dimension up to a multiple of 32.
+   The arrays feed cmcx_domain (h = spacing, base_t). */
+#define CMCX_SHAPE3D 3
+#define CMCX_SHAPE2D 2
+typedef struct cmcx_geometry cmcx_geometry;
+int cmcx_geometry_load(int kind, const char* path, const double* spacing, double base_t, double depth,
+                       double depth_var, int align, cmcx_geometry** out);
+/* Grid dimensions, the output box (metres; an extruded shape spans z in
+   [-depth, 0]), the number of frames and the cycle length (s).  Any pointer may
+   be NULL. */
+int cmcx_geometry_info(cmcx_geometry* g, int32_t* dims, double* bbox_min, double* bbox_max, int* nframes,
+                       double* cycle);
+/* Builds the grid of the frame at `time` (frames blended as the reference's
+   Prepare) into host arrays of dims[0]*dims[1]*dims[2] elements (any may be NULL). */
+int cmcx_geometry_build(cmcx_geometry* g, double time, uint8_t* type, uint8_t* bc_vel, uint8_t* bc_temp,
+                        double* vx, double* vy, double* vz, double* T);
+int cmcx_geometry_destroy(cmcx_geometry* g);
 
 /* ---- COARDS result files (the reference's netCDF output, IO.h:136-388) -----
    Written as netCDF classic with 64-bit offsets (no netCDF library needed),

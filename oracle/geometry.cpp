@@ -1,0 +1,568 @@
+// ORACLE -- test infrastructure only.
+//
+// CPU restatement of the reference's geometry ingestion (samech-git/cmc-fluid-solver):
+// the Shape3D triangle-mesh and Shape2D polyline loaders and the builds that
+// turn them into the node arrays of a 3D grid.  It is the checker the HIP
+// product (cmc-fluid-solver_amd/csrc/geometry.hip) is compared against; only
+// tests/ may load it.  FTYPE is float, as the reference's default build
+// (src/Common/Geometry.h:21); -ffp-contract=off keeps its operation order.
+//
+// Follows, function by function:
+//   read_point2d/3d     src/Common/IO.h:511-582 (ReadPoint2D, ReadPoint3D; the
+//                       3D reader drops the first character of the token that
+//                       follows a point on the same line, as the reference;
+//                       CRLF line ends read as text mode reads them)
+//   load3d              src/FluidSolver3D/Grid3D.cpp:373-436 (Load3DShape),
+//                       BBox3D::Build src/Common/Geometry.h:510-529, Init :353-371
+//   raster_polygon      Grid3D.cpp:722-801 (RasterPolygon, ProjectPointOnPolygon
+//                       :702-720, GetIntersectHorizon :690-700)
+//   raster_line3d       Grid3D.cpp:803-821 (RasterLine)
+//   flood_fill3d        Grid3D.cpp:823-862 (FloodFill)
+//   build3d             Grid3D.cpp:864-912 (Build), frame blend :914-960, time
+//                       to frame Grid3D.cpp:962-981 (Prepare3D_Shape)
+//   load2d              src/FluidSolver2D/Grid2D.cpp:272-356 (LoadFromFile),
+//                       ComputeBorderVelocities :358-379, Init :207-239
+//   raster_line2d       Grid2D.cpp:105-141 (RasterLine, bc_noslip branch)
+//   flood_fill2d        Grid2D.cpp:157-197
+//   build2d             Grid2D.cpp:241-270 (Build; no field file), frame blend
+//                       :382-422, Prepare(time) :454-478
+//   extrude             src/FluidSolver3D/Grid3D.cpp:608-666 (Prepare2D),
+//                       LoadFromFile :488-514
+//
+// Node data the reference leaves unset: Init allocates nodes with new[] and
+// sets type, v and T only, so BOUND nodes of a triangle-mesh grid keep bc_vel
+// and bc_temp from fresh memory; they are taken as zero (BC_NOSLIP) here.
+//
+// What pins it (tests/test_geometry.py): the box meshes of the reference's
+// example data give analytic masks (walls on the grid planes of the box faces,
+// fluid inside, NODE_OUT outside); masks are well formed; every grid built is
+// invariant under the frame time of a one-frame file; the extruded columns
+// follow Prepare2D's layer rule (NODE_OUT, bottom wall, fluid, top wall,
+// NODE_OUT) cell by cell.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace orcg {
+
+typedef float F;
+enum { IN = 0, OUT = 1, BOUND = 2, VALVE = 3 };
+enum { NOSLIP = 0, FREE = 1 };
+const double EPS = 1e-8;       // COMP_EPS
+const double PAD = 0.02;       // BBOX_PADDING
+const F SCALE = 0.001f;        // GRID_SCALE_FACTOR
+const double BIG = 1e10;       // INF
+
+struct V2 { F x = 0, y = 0; };
+struct V3 { F x = 0, y = 0, z = 0; };
+
+// One character as a text-mode read sees it: the reference's files carry
+// CRLF line ends and were read in Windows text mode, where "\r\n" arrives as
+// "\n"; a '\r' is skipped.
+static bool read_char(FILE* f, char& c) {
+    do {
+        if (fscanf(f, "%c", &c) != 1) return false;
+    } while (c == '\r');
+    return true;
+}
+
+// ReadPoint2D: the rest of the line, ',' -> '.', split at the first space
+static V2 read_point2d(FILE* f) {
+    std::string s;
+    char c = 0;
+    read_char(f, c);
+    while (c == '\n' || c == ' ') if (!read_char(f, c)) break;
+    while (c != '\n') { s += c; if (!read_char(f, c)) break; }
+    for (auto& ch : s) if (ch == ',') ch = '.';
+    size_t k = s.find(' ');
+    V2 p;
+    p.x = (F)atof(s.substr(0, k).c_str());
+    p.y = (F)atof(k == std::string::npos ? "" : s.substr(k + 1).c_str());
+    return p;
+}
+
+// ReadPoint3D: three tokens; after the third it reads one character past the
+// spaces that follow (lost to the next read)
+static V3 read_point3d(FILE* f) {
+    std::string s[3];
+    char c = 0;
+    read_char(f, c);
+    while (c == '\n' || c == ' ') if (!read_char(f, c)) break;
+    for (int i = 0; i < 3; i++) {
+        while (c != '\n' && c != ' ') { s[i] += c; if (!read_char(f, c)) { c = '\n'; break; } }
+        size_t k = s[i].find(',');
+        if (k != std::string::npos) s[i][k] = '.';
+        while (c == ' ') if (!read_char(f, c)) break;
+    }
+    V3 p;
+    p.x = (F)atof(s[0].c_str());
+    p.y = (F)atof(s[1].c_str());
+    p.z = (F)atof(s[2].c_str());
+    return p;
+}
+
+static int align32(int n) { return (n & 31) == 0 ? n : ((n >> 5) + 1) << 5; }
+
+// ---------------------------------------------------------------- 3D mesh
+struct Mesh { std::vector<V3> v, vel; std::vector<int> idx; };
+
+struct Grid {
+    int nx = 0, ny = 0, nz = 0;
+    double dx, dy, dz, baseT;
+    std::vector<uint8_t> type, bcv, bct;
+    std::vector<F> vx, vy, vz, T;
+    size_t id(int i, int j, int k) const { return ((size_t)i * ny + j) * nz + k; }
+    void alloc() {
+        size_t n = (size_t)nx * ny * nz;
+        type.assign(n, OUT); bcv.assign(n, 0); bct.assign(n, 0);
+        vx.assign(n, 0); vy.assign(n, 0); vz.assign(n, 0); T.assign(n, 0);
+    }
+    void set_type(int i, int j, int k, int t) { type[id(i, j, k)] = (uint8_t)t; }
+    void set_data(size_t q, int bv, int bt, F ux, F uy, F uz, F t) {
+        bcv[q] = (uint8_t)bv; bct[q] = (uint8_t)bt; vx[q] = ux; vy[q] = uy; vz[q] = uz; T[q] = t;
+    }
+};
+
+struct Geo3 {
+    Grid g;
+    std::vector<Mesh> frames;
+    std::vector<double> dur;
+    V3 bmin, bmax;
+};
+
+static bool load3d(const char* path, Geo3& G, bool align) {
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    int nf = 0;
+    if (fscanf(f, "%i", &nf) != 1 || nf < 1) { fclose(f); return false; }
+    G.frames.resize(nf);
+    G.dur.assign(nf, 1.0 / 75);
+    for (auto& m : G.frames) {
+        int nv = 0, ni = 0;
+        if (fscanf(f, "%i", &nv) != 1) { fclose(f); return false; }
+        m.v.resize(nv); m.vel.resize(nv);
+        for (int k = 0; k < nv; k++) {
+            V3 p = read_point3d(f);
+            m.v[k] = V3{p.x * SCALE, p.y * SCALE, p.z * SCALE};
+            m.vel[k] = read_point3d(f);
+        }
+        if (fscanf(f, "%i", &ni) != 1) { fclose(f); return false; }
+        m.idx.resize(3 * ni);
+        for (int k = 0; k < 3 * ni; k++)
+            if (fscanf(f, "%i", &m.idx[k]) != 1) { fclose(f); return false; }
+    }
+    fclose(f);
+    // BBox3D::Build
+    F mn[3] = {(F)BIG, (F)BIG, (F)BIG}, mx[3] = {(F)-BIG, (F)-BIG, (F)-BIG};
+    for (auto& m : G.frames)
+        for (auto& p : m.v) {
+            F c[3] = {p.x, p.y, p.z};
+            for (int a = 0; a < 3; a++) { if (c[a] < mn[a]) mn[a] = c[a]; if (c[a] > mx[a]) mx[a] = c[a]; }
+        }
+    for (int a = 0; a < 3; a++) {
+        F w = mx[a] - mn[a];
+        mn[a] -= w * (F)PAD;
+        mx[a] += w * (F)PAD;
+    }
+    G.bmin = V3{mn[0], mn[1], mn[2]};
+    G.bmax = V3{mx[0], mx[1], mx[2]};
+    Grid& g = G.g;
+    g.nx = (int)ceil((mx[0] - mn[0]) / g.dx) + 1;
+    g.ny = (int)ceil((mx[1] - mn[1]) / g.dy) + 1;
+    g.nz = (int)ceil((mx[2] - mn[2]) / g.dz) + 1;
+    if (align) { g.nx = align32(g.nx); g.ny = align32(g.ny); g.nz = align32(g.nz); }
+    g.alloc();
+    for (auto& m : G.frames)
+        for (auto& p : m.v) {
+            p.x = (p.x - mn[0]) / (F)g.dx;
+            p.y = (p.y - mn[1]) / (F)g.dy;
+            p.z = (p.z - mn[2]) / (F)g.dz;
+        }
+    return true;
+}
+
+static V2 horizon(V2 p1, V2 p2, V2 p) {
+    V2 r;
+    r.y = p.y;
+    if (fabs(p1.y - p2.y) < EPS) r.x = p.x;
+    else r.x = p1.x + (p2.x - p1.x) * (r.y - p1.y) / (p2.y - p1.y);
+    return r;
+}
+
+static F dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+
+static void project(Grid& g, int dir, int i, int j, V2 tp, V3 n, F d) {
+    int k;
+    switch (dir) {
+        case 0: k = (int)((-d - dot2(tp, V2{n.y, n.z})) / n.x);
+            if (k >= 0 && k < g.nx) g.set_type(k, i, j, BOUND); break;
+        case 1: k = (int)((-d - dot2(tp, V2{n.x, n.z})) / n.y);
+            if (k >= 0 && k < g.ny) g.set_type(i, k, j, BOUND); break;
+        default: k = (int)((-d - dot2(tp, V2{n.x, n.y})) / n.z);
+            if (k >= 0 && k < g.nz) g.set_type(i, j, k, BOUND); break;
+    }
+}
+
+static bool equal3(V3 a, V3 b) {
+    return fabs(a.x - b.x) < EPS && fabs(a.y - b.y) < EPS && fabs(a.z - b.z) < EPS;
+}
+
+static void raster_polygon(Grid& g, V3 p1, V3 p2, V3 p3) {
+    if (equal3(p1, p2) && equal3(p1, p3)) return;
+    V3 e1{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z}, e2{p3.x - p1.x, p3.y - p1.y, p3.z - p1.z};
+    V3 n{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+    F len = (F)sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    F t = 1 / len;
+    n.x *= t; n.y *= t; n.z *= t;
+    F d = -(p1.x * n.x + p1.y * n.y + p1.z * n.z);
+    F maxv = std::max(std::fabs(n.x), std::max(std::fabs(n.y), std::fabs(n.z)));
+    int dir = 0;
+    if (fabs(maxv - std::fabs(n.x)) < EPS) dir = 0;
+    if (fabs(maxv - std::fabs(n.y)) < EPS) dir = 1;
+    if (fabs(maxv - std::fabs(n.z)) < EPS) dir = 2;
+    V2 a, b, c;
+    switch (dir) {
+        case 0: a = {p1.y, p1.z}; b = {p2.y, p2.z}; c = {p3.y, p3.z}; break;
+        case 1: a = {p1.x, p1.z}; b = {p2.x, p2.z}; c = {p3.x, p3.z}; break;
+        default: a = {p1.x, p1.y}; b = {p2.x, p2.y}; c = {p3.x, p3.y}; break;
+    }
+    V2 m;
+    if (c.y < b.y) { m = c; c = b; b = m; }
+    if (a.y > b.y) { m = a; a = b; b = m; }
+    if (c.y < b.y) { m = c; c = b; b = m; }
+    m = horizon(a, c, b);
+    V2 d1{m.x - a.x, m.y - a.y}, d2{c.x - m.x, c.y - m.y};
+    int s1 = (int)std::max(std::fabs(d1.x), std::fabs(d1.y)) + 1;
+    int s2 = (int)std::max(std::fabs(d2.x), std::fabs(d2.y)) + 1;
+    V2 dp1{d1.x / s1, d1.y / s1}, dp2{d2.x / s2, d2.y / s2};
+    V2 p = a;
+    int di = (m.x < b.x) ? 1 : -1;
+    for (; p.y < m.y;) {
+        int j = (int)p.y;
+        int last = (int)horizon(a, b, p).x;
+        for (int i = (int)p.x; i != last + di; i += di) project(g, dir, i, j, V2{(F)i, p.y}, n, d);
+        p.x += dp1.x; p.y += dp1.y;
+    }
+    for (; p.y < c.y;) {
+        int j = (int)p.y;
+        int last = (int)horizon(b, c, p).x;
+        for (int i = (int)p.x; i != last + di; i += di) project(g, dir, i, j, V2{(F)i, p.y}, n, d);
+        p.x += dp2.x; p.y += dp2.y;
+    }
+}
+
+static void raster_line3d(Grid& g, V3 p1, V3 p2) {
+    V3 dir{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+    int steps = (int)std::max(std::fabs(dir.x), std::max(std::fabs(dir.y), std::fabs(dir.z))) + 1;
+    F s = 1 / (F)steps;
+    V3 dp{dir.x * s, dir.y * s, dir.z * s};
+    V3 p = p1;
+    for (int i = 0; i <= steps; i++) {
+        g.set_type((int)p.x, (int)p.y, (int)p.z, BOUND);
+        p.x += dp.x; p.y += dp.y; p.z += dp.z;
+    }
+}
+
+static void flood3d(Grid& g, int si, int sj, int sk, int color) {
+    std::vector<int> q;
+    q.push_back(si); q.push_back(sj); q.push_back(sk);
+    g.set_type(si, sj, sk, color);
+    const int nb[18] = {-1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1, 0, 0, 0, -1, 0, 0, 1};
+    for (size_t cur = 0; cur < q.size() / 3; cur++) {
+        int i = q[3 * cur], j = q[3 * cur + 1], k = q[3 * cur + 2];
+        for (int t = 0; t < 6; t++) {
+            int a = i + nb[3 * t], b = j + nb[3 * t + 1], c = k + nb[3 * t + 2];
+            if (a >= 0 && a < g.nx && b >= 0 && b < g.ny && c >= 0 && c < g.nz && g.type[g.id(a, b, c)] == IN) {
+                q.push_back(a); q.push_back(b); q.push_back(c);
+                g.set_type(a, b, c, color);
+            }
+        }
+    }
+}
+
+// frame index and blend weight of `time` (Prepare3D_Shape / Grid2D::Prepare)
+static void frame_at(const std::vector<double>& dur, double time, int& frame, double& sub) {
+    std::vector<double> a(dur.size() + 1, 0.0);
+    for (size_t i = 1; i <= dur.size(); i++) a[i] = a[i - 1] + dur[i - 1];
+    double r = fmod(time, a[dur.size()]);
+    frame = 0;
+    for (size_t i = 1; i < dur.size(); i++) if (a[i] < r) frame = (int)i;
+    sub = (r - a[frame]) / (a[frame + 1] - a[frame]);
+}
+
+static void build3d(Geo3& G, double time) {
+    int fr; double sub;
+    frame_at(G.dur, time, fr, sub);
+    const Mesh& m0 = G.frames[fr % G.frames.size()];
+    const Mesh& m1 = G.frames[(fr + 1) % G.frames.size()];
+    const F ss = (F)sub, is = 1 - ss;  // ComputeSubframeInfo takes the weight as FTYPE
+    std::vector<V3> v(m0.v.size());
+    for (size_t k = 0; k < v.size(); k++)
+        v[k] = V3{m0.v[k].x * is + m1.v[k].x * ss, m0.v[k].y * is + m1.v[k].y * ss, m0.v[k].z * is + m1.v[k].z * ss};
+    Grid& g = G.g;
+    std::fill(g.type.begin(), g.type.end(), (uint8_t)IN);
+    for (size_t t = 0; t + 2 < m0.idx.size(); t += 3) {
+        V3 a = v[m0.idx[t]], b = v[m0.idx[t + 1]], c = v[m0.idx[t + 2]];
+        raster_polygon(g, a, b, c);
+        raster_line3d(g, a, b);
+        raster_line3d(g, a, c);
+        raster_line3d(g, c, b);
+    }
+    flood3d(g, 0, 0, 0, OUT);
+    for (size_t q = 0; q < g.type.size(); q++)
+        if (g.type[q] == IN || g.type[q] == OUT) g.set_data(q, NOSLIP, NOSLIP, 0, 0, 0, (F)g.baseT);
+}
+
+// ---------------------------------------------------------------- 2D shape
+struct Poly { std::vector<V2> p, vel; bool active; };
+struct Frame2 { double dur; std::vector<Poly> shapes; };
+
+struct Cell { int cell = OUT; int type = NOSLIP; V2 vel; F T = 0; };
+
+struct Geo2 {
+    std::vector<Frame2> frames;
+    int nx = 0, ny = 0;
+    double dx, dy, startT;
+    V2 bmin, bmax;
+    std::vector<Cell> cur;
+    // 3D extrusion
+    Grid g;
+    double depth, depth_var;
+    int active_nz = 0;
+};
+
+static bool load2d(const char* path, Geo2& G, bool align) {
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    int nf = 0;
+    if (fscanf(f, "%i", &nf) != 1 || nf < 1) { fclose(f); return false; }
+    G.frames.resize(nf);
+    for (auto& fr : G.frames) {
+        float dur; int ns;
+        if (fscanf(f, "%f", &dur) != 1 || fscanf(f, "%i", &ns) != 1) { fclose(f); return false; }
+        fr.dur = dur;
+        fr.shapes.resize(ns);
+        for (auto& s : fr.shapes) {
+            int np;
+            if (fscanf(f, "%i", &np) != 1) { fclose(f); return false; }
+            s.p.resize(np); s.vel.resize(np);
+            for (int k = 0; k < np; k++) { V2 p = read_point2d(f); s.p[k] = V2{p.x * SCALE, p.y * SCALE}; }
+            char str[8] = {0};
+            if (fscanf(f, "%7s", str) != 1) { fclose(f); return false; }
+            V2 v;
+            if (str[0] == 'M') { s.active = true; v = read_point2d(f); } else s.active = false;
+            for (int k = 0; k < np; k++) s.vel[k] = V2{v.x * SCALE, v.y * SCALE};
+        }
+    }
+    fclose(f);
+    // ComputeBorderVelocities
+    for (int fr = 0; fr < nf; fr++) {
+        int nxt = (fr + 1) % nf;
+        double m = 1 / G.frames[fr].dur;
+        for (size_t i = 0; i < G.frames[fr].shapes.size(); i++) {
+            auto& a = G.frames[fr].shapes[i];
+            auto& b = G.frames[nxt].shapes[i];
+            for (size_t k = 0; k < a.p.size(); k++) {
+                V2 dd = a.active ? V2{a.p[k].x - b.p[k].x, a.p[k].y - b.p[k].y} : V2{b.p[k].x - a.p[k].x, b.p[k].y - a.p[k].y};
+                V2 w{dd.x * (F)m, dd.y * (F)m};
+                if (a.active) { b.vel[k].x += w.x; b.vel[k].y += w.y; } else b.vel[k] = w;
+            }
+        }
+    }
+    // Init: BBox2D::Build, dims, grid coordinates
+    F mn[2] = {(F)BIG, (F)BIG}, mx[2] = {(F)-BIG, (F)-BIG};
+    for (auto& fr : G.frames)
+        for (auto& s : fr.shapes)
+            for (auto& p : s.p) {
+                if (p.x < mn[0]) mn[0] = p.x; if (p.y < mn[1]) mn[1] = p.y;
+                if (p.x > mx[0]) mx[0] = p.x; if (p.y > mx[1]) mx[1] = p.y;
+            }
+    for (int a = 0; a < 2; a++) { F w = mx[a] - mn[a]; mn[a] -= w * (F)PAD; mx[a] += w * (F)PAD; }
+    G.bmin = V2{mn[0], mn[1]};
+    G.bmax = V2{mx[0], mx[1]};
+    G.nx = (int)ceil((mx[0] - mn[0]) / G.dx) + 1;
+    G.ny = (int)ceil((mx[1] - mn[1]) / G.dy) + 1;
+    if (align) { G.nx = align32(G.nx); G.ny = align32(G.ny); }
+    G.cur.assign((size_t)G.nx * G.ny, Cell());
+    for (auto& fr : G.frames)
+        for (auto& s : fr.shapes)
+            for (auto& p : s.p) { p.x = (p.x - mn[0]) / (F)G.dx; p.y = (p.y - mn[1]) / (F)G.dy; }
+    return true;
+}
+
+static void raster_line2d(Geo2& G, V2 p1, V2 p2, V2 v1, V2 v2, int color) {
+    V2 o{p2.x - p1.x, p2.y - p1.y};
+    int steps = (int)std::max(std::fabs(o.x), std::fabs(o.y)) + 1;
+    V2 dp{o.x / steps, o.y / steps};
+    V2 dv{(v2.x - v1.x) / steps, (v2.y - v1.y) / steps};
+    V2 p = p1, v = v1;
+    for (int i = 0; i <= steps; i++) {
+        int x = (int)p.x, y = (int)p.y;
+        Cell& c = G.cur[(size_t)x * G.ny + y];
+        c.type = NOSLIP; c.cell = color; c.vel = v; c.T = (F)G.startT;
+        p.x += dp.x; p.y += dp.y;
+        v.x += dv.x; v.y += dv.y;
+    }
+}
+
+static void flood2d(Geo2& G, int color) {
+    std::vector<int> q{0, 0};
+    G.cur[0].cell = color;
+    const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (size_t cur = 0; cur < q.size() / 2; cur++) {
+        int i = q[2 * cur], j = q[2 * cur + 1];
+        for (int k = 0; k < 4; k++) {
+            int a = i + nb[k][0], b = j + nb[k][1];
+            if (a >= 0 && a < G.nx && b >= 0 && b < G.ny && G.cur[(size_t)a * G.ny + b].cell == IN) {
+                q.push_back(a); q.push_back(b);
+                G.cur[(size_t)a * G.ny + b].cell = color;
+            }
+        }
+    }
+}
+
+static void build2d(Geo2& G, double time) {
+    std::vector<double> dur;
+    for (auto& fr : G.frames) dur.push_back(fr.dur);
+    int fr; double sub;
+    frame_at(dur, time, fr, sub);
+    const Frame2& f0 = G.frames[fr % G.frames.size()];
+    const Frame2& f1 = G.frames[(fr + 1) % G.frames.size()];
+    F is = (F)(1 - sub), ss = (F)sub;
+    std::vector<Poly> sh(f0.shapes.size());
+    for (size_t i = 0; i < sh.size(); i++) {
+        sh[i].active = f0.shapes[i].active;
+        for (size_t k = 0; k < f0.shapes[i].p.size(); k++) {
+            const V2 a = f0.shapes[i].p[k], b = f1.shapes[i].p[k];
+            const V2 va = f0.shapes[i].vel[k], vb = f1.shapes[i].vel[k];
+            V2 p{a.x * is + b.x * ss, a.y * is + b.y * ss};
+            p.y = a.y * is + b.y * ss;
+            sh[i].p.push_back(p);
+            sh[i].vel.push_back(V2{va.x * is + vb.x * ss, va.y * is + vb.y * ss});
+        }
+    }
+    for (auto& c : G.cur) c.cell = IN;
+    for (int pass = 0; pass < 2; pass++)  // VALVE lines, then BOUND lines
+        for (auto& s : sh)
+            if (s.active == (pass == 0))
+                for (size_t i = 0; i + 1 < s.p.size(); i++)
+                    raster_line2d(G, s.p[i], s.p[i + 1], s.vel[i], s.vel[i + 1], pass == 0 ? VALVE : BOUND);
+    flood2d(G, OUT);
+    for (auto& c : G.cur)
+        if (c.cell == IN || c.cell == OUT) { c.type = NOSLIP; c.vel = V2(); c.T = (F)G.startT; }
+}
+
+static void extrude(Geo2& G) {
+    Grid& g = G.g;
+    const int nz = g.nz, az = G.active_nz;
+    std::fill(g.type.begin(), g.type.end(), (uint8_t)IN);
+    std::fill(g.bcv.begin(), g.bcv.end(), (uint8_t)0);
+    std::fill(g.bct.begin(), g.bct.end(), (uint8_t)0);
+    std::fill(g.vx.begin(), g.vx.end(), (F)0); std::fill(g.vy.begin(), g.vy.end(), (F)0);
+    std::fill(g.vz.begin(), g.vz.end(), (F)0); std::fill(g.T.begin(), g.T.end(), (F)0);
+    auto bound = [&](size_t q, int bv, int bt, F ux, F uy, F t) {
+        g.type[q] = BOUND; g.set_data(q, bv, bt, ux, uy, 0, t);
+    };
+    for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++) {
+            const Cell& c = G.cur[(size_t)i * G.ny + j];
+            if (c.cell == OUT) {
+                for (int k = 0; k < nz; k++) g.type[g.id(i, j, k)] = OUT;
+                continue;
+            }
+            for (int k = az - 1; k < nz; k++) g.type[g.id(i, j, k)] = OUT;
+            bound(g.id(i, j, az - 2), NOSLIP, FREE, 0, 0, (F)g.baseT);
+            int height = std::max(az - 2 - 2, 0);
+            double x = -1 + 2 * (double)i / g.nx;
+            double y = -1 + 2 * (double)j / g.ny;
+            double z = 1.0 - (x * x + y * y) * 0.5;
+            int bottom = 1 + (int)(G.depth_var * z * height);
+            g.type[g.id(i, j, 0)] = OUT;
+            for (int k = 1; k <= bottom; k++) bound(g.id(i, j, k), NOSLIP, FREE, 0, 0, (F)g.baseT);
+            for (int k = bottom + 1; k < az - 2; k++) {
+                size_t q = g.id(i, j, k);
+                switch (c.cell) {
+                    case BOUND: bound(q, NOSLIP, FREE, c.vel.x, c.vel.y, c.T); break;
+                    case VALVE:
+                        if (c.vel.x == 0 && c.vel.y == 0) bound(q, FREE, FREE, c.vel.x, c.vel.y, c.T);
+                        else bound(q, NOSLIP, NOSLIP, c.vel.x, c.vel.y, c.T);
+                        g.type[q] = VALVE;
+                        break;
+                    case IN: g.type[q] = IN; g.T[q] = (F)g.baseT; break;
+                }
+            }
+        }
+}
+
+}  // namespace orcg
+
+using namespace orcg;
+
+struct OrcGeo {
+    int kind;  // 3 = triangle mesh, 2 = extruded 2D shape
+    Geo3 g3;
+    Geo2 g2;
+    Grid& grid() { return kind == 3 ? g3.g : g2.g; }
+};
+
+extern "C" {
+
+void* orc_geo_load(int kind, const char* path, double dx, double dy, double dz, double depth,
+                   double depth_var, double base_t, int align) {
+    OrcGeo* h = new OrcGeo();
+    h->kind = kind;
+    bool ok;
+    if (kind == 3) {
+        Grid& g = h->g3.g;
+        g.dx = dx; g.dy = dy; g.dz = dz; g.baseT = base_t;
+        ok = load3d(path, h->g3, align != 0);
+    } else {
+        Geo2& G = h->g2;
+        G.dx = dx; G.dy = dy; G.startT = base_t; G.depth = depth; G.depth_var = depth_var;
+        ok = load2d(path, G, align != 0);
+        if (ok) {
+            Grid& g = G.g;
+            g.dx = dx; g.dy = dy; g.dz = dz; g.baseT = base_t;
+            g.nx = G.nx; g.ny = G.ny;
+            G.active_nz = (int)ceil(depth / dz) + 1;
+            g.nz = align ? align32(G.active_nz) : G.active_nz;
+            g.alloc();
+        }
+    }
+    if (!ok) { delete h; return nullptr; }
+    return h;
+}
+
+void orc_geo_dims(void* p, int* dims) {
+    Grid& g = static_cast<OrcGeo*>(p)->grid();
+    dims[0] = g.nx; dims[1] = g.ny; dims[2] = g.nz;
+}
+
+void orc_geo_build(void* p, double time, uint8_t* type, uint8_t* bcv, uint8_t* bct, double* vx,
+                   double* vy, double* vz, double* T) {
+    OrcGeo* h = static_cast<OrcGeo*>(p);
+    if (h->kind == 3) build3d(h->g3, time);
+    else { build2d(h->g2, time); extrude(h->g2); }
+    Grid& g = h->grid();
+    for (size_t q = 0; q < g.type.size(); q++) {
+        type[q] = g.type[q]; bcv[q] = g.bcv[q]; bct[q] = g.bct[q];
+        vx[q] = g.vx[q]; vy[q] = g.vy[q]; vz[q] = g.vz[q]; T[q] = g.T[q];
+    }
+}
+
+// the 2D grid of a Shape2D load (after orc_geo_build): cell types
+void orc_geo_grid2d(void* p, int* dims, uint8_t* cells) {
+    OrcGeo* h = static_cast<OrcGeo*>(p);
+    dims[0] = h->g2.nx; dims[1] = h->g2.ny;
+    if (cells) for (size_t q = 0; q < h->g2.cur.size(); q++) cells[q] = (uint8_t)h->g2.cur[q].cell;
+}
+
+void orc_geo_free(void* p) { delete static_cast<OrcGeo*>(p); }
+
+}  // extern "C"

@@ -83,6 +83,13 @@ def load():
     L.orc2_get_layer_raw.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_d, _c_d, _c_d]
     L.orc2_segments.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_i32]
     L.orc2_segments.restype = ctypes.c_int
+    # geometry ingestion
+    L.orc_geo_load.restype = ctypes.c_void_p
+    L.orc_geo_load.argtypes = [ctypes.c_int, ctypes.c_char_p] + [ctypes.c_double] * 6 + [ctypes.c_int]
+    L.orc_geo_dims.argtypes = [ctypes.c_void_p, _c_i32]
+    L.orc_geo_build.argtypes = [ctypes.c_void_p, ctypes.c_double, _c_u8, _c_u8, _c_u8, _c_d, _c_d, _c_d, _c_d]
+    L.orc_geo_grid2d.argtypes = [ctypes.c_void_p, _c_i32, _c_u8]
+    L.orc_geo_free.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -260,3 +267,47 @@ class Oracle2D:
         out = np.zeros((max(n, 1), 5), np.int32)
         self.L.orc2_segments(self.h, axis, _ptr(out, _c_i32))
         return out[:n]
+
+
+class OracleGeometry:
+    """CPU restatement of the reference's geometry loaders (Shape3D triangle
+    meshes, Shape2D polylines extruded in depth)."""
+
+    def __init__(self, kind: str, path: str, dx, dy, dz, base_t=1.0, depth=0.0, depth_var=0.0,
+                 align=False):
+        L = load()
+        self.L = L
+        k = 3 if kind == "shape3d" else 2
+        self.h = L.orc_geo_load(k, str(path).encode(), dx, dy, dz, depth, depth_var, base_t,
+                                1 if align else 0)
+        if not self.h:
+            raise RuntimeError(f"oracle could not load {path}")
+        d = np.zeros(3, np.int32)
+        L.orc_geo_dims(self.h, _ptr(d, _c_i32))
+        self.shape = tuple(int(x) for x in d)
+
+    def build(self, time=0.0):
+        n = int(np.prod(self.shape))
+        t, bv, bt = (np.zeros(n, np.uint8) for _ in range(3))
+        vx, vy, vz, T = (np.zeros(n) for _ in range(4))
+        self.L.orc_geo_build(self.h, time, *[_ptr(a, _c_u8) for a in (t, bv, bt)],
+                             *[_ptr(a, _c_d) for a in (vx, vy, vz, T)])
+        return t, bv, bt, vx, vy, vz, T
+
+    def grid2d(self):
+        d = np.zeros(2, np.int32)
+        self.L.orc_geo_grid2d(self.h, _ptr(d, _c_i32), ctypes.cast(None, _c_u8))
+        c = np.zeros(int(d[0] * d[1]), np.uint8)
+        self.L.orc_geo_grid2d(self.h, _ptr(d, _c_i32), _ptr(c, _c_u8))
+        return c.reshape(int(d[0]), int(d[1]))
+
+    def close(self):
+        if self.h:
+            self.L.orc_geo_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
