@@ -106,6 +106,7 @@ SIGNATURES = [
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("cmcx_geometry_info", ctypes.c_int, [_vp, _i32p, _dp, _dp, ctypes.POINTER(ctypes.c_int), _dp]),
     ("cmcx_geometry_build", ctypes.c_int, [_vp, ctypes.c_double, _u8p, _u8p, _u8p, _dp, _dp, _dp, _dp]),
+    ("cmcx_geometry_build2d", ctypes.c_int, [_vp, ctypes.c_double, _u8p, _u8p, _dp, _dp, _dp]),
     ("cmcx_geometry_destroy", ctypes.c_int, [_vp]),
     ("cmcx_coards_write_header", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cmcx_coards_header)]),
     ("cmcx_coards_write_layer", ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int, _dp, ctypes.c_int,
@@ -117,6 +118,11 @@ SIGNATURES = [
                                    ctypes.POINTER(ctypes.c_int)]),
     ("cmcx2d_get_fields", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("cmcx2d_destroy", ctypes.c_int, [_vp]),
+    ("cmcx2d_create_method", ctypes.c_int, [ctypes.POINTER(cmcx2d_domain), ctypes.POINTER(cmcx_params),
+                                            ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    ("cmcx2d_set_poisson", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int64]),
+    ("cmcx2d_set_grid", ctypes.c_int, [_vp, _u8p, _u8p, _dp, _dp, _dp]),
+    ("cmcx2d_poisson_sweeps", ctypes.c_int, [_vp, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 ]
 
 _lib = None

@@ -279,6 +279,8 @@ class AdiSolver2D:
     cmcx._lib.CmcxError with CMCX_ERR_PICARD where the reference exits), plus
     fields() = (u, v, p, T)."""
 
+    method = 1  # CMCX2D_ADI
+
     def __init__(self, dtype: str = "f64"):
         self.h = ctypes.c_void_p(None)
         self.dtype = _lib.CMCX_F32 if dtype in ("f32", "float32", np.float32) else _lib.CMCX_F64
@@ -298,8 +300,8 @@ class AdiSolver2D:
         dom.T = vals[2].ctypes.data_as(_lib._dp)
         prm = _lib.cmcx_params(params.v_T, params.v_vis, params.t_vis, params.t_phi)
         h = ctypes.c_void_p(None)
-        check(lib().cmcx2d_create(ctypes.byref(dom), ctypes.byref(prm), self.dtype,
-                                  ctypes.c_void_p(stream), ctypes.byref(h)), "cmcx2d_create")
+        check(lib().cmcx2d_create_method(ctypes.byref(dom), ctypes.byref(prm), self.method, self.dtype,
+                                         ctypes.c_void_p(stream), ctypes.byref(h)), "cmcx2d_create_method")
         self.h = h
         self.grid = grid
         self.params = params
@@ -307,6 +309,13 @@ class AdiSolver2D:
 
     def UpdateBoundaries(self):
         check(lib().cmcx2d_update_boundaries(self.h), "cmcx2d_update_boundaries")
+
+    def SetGrid(self, t, bc, vx, vy, T):
+        """Grid2D::Prepare(t) between steps: new cell types, BC kinds and node values."""
+        arrs = [_u8(t), _u8(bc)]
+        vals = [_f64(vx), _f64(vy), _f64(T)]
+        check(lib().cmcx2d_set_grid(self.h, *[a.ctypes.data_as(_lib._u8p) for a in arrs],
+                                    *[a.ctypes.data_as(_lib._dp) for a in vals]), "cmcx2d_set_grid")
 
     def TimeStep(self, dt: float, num_global: int, num_local: int = 1):
         err = ctypes.c_double(0.0)
@@ -331,6 +340,28 @@ class AdiSolver2D:
             self.close()
         except Exception:
             pass
+
+
+class StableSolver2D(AdiSolver2D):
+    """The reference's 2D Stable solver (src/FluidSolver2D/StableSolver2D.h):
+    explicit advection-diffusion of u and v, then a projection whose Poisson
+    equation is relaxed on the GPU in the reference's cell order.  Same
+    interface as AdiSolver2D (num_local is unused, as in the reference), plus
+    the relaxation's settings and sweep counts."""
+
+    method = 2  # CMCX2D_STABLE
+
+    def set_poisson(self, tol: float = 1e-2, max_sweeps: int = 1000000):
+        check(lib().cmcx2d_set_poisson(self.h, float(tol), int(max_sweeps)), "cmcx2d_set_poisson")
+
+    def poisson_sweeps(self):
+        """Relaxation sweeps of each project of the last step."""
+        n = ctypes.c_int(0)
+        check(lib().cmcx2d_poisson_sweeps(self.h, None, 0, ctypes.byref(n)), "cmcx2d_poisson_sweeps")
+        out = np.zeros(max(n.value, 1), np.int32)
+        check(lib().cmcx2d_poisson_sweeps(self.h, out.ctypes.data_as(_lib._i32p), n.value, ctypes.byref(n)),
+              "cmcx2d_poisson_sweeps")
+        return out[:n.value]
 
 
 def _domain_struct(grid, keep: list):

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -442,7 +443,6 @@ struct Geometry {
         bcv = dalloc<uint8_t>(n);
         bct = dalloc<uint8_t>(n);
         for (float*& p : fv) p = dalloc<float>(n);
-        d_changed = dalloc<int>(1);
         // Init: NODE_OUT, v = 0, T = 0 (BC kinds zero: BC_NOSLIP)
         hipLaunchKernelGGL(fill_u8_kernel, dim3(nblk(n)), dim3(256), 0, 0, type, n, (uint8_t)NODE_OUT);
         CMCX_HIP(hipMemset(bcv, 0, n));
@@ -464,6 +464,7 @@ struct Geometry {
 
     // iterated flood fill from node 0 over a grid of nx x ny x nz (nz = 1: 2D)
     void flood(uint8_t* t, Box b) {
+        if (!d_changed) d_changed = dalloc<int>(1);
         hipLaunchKernelGGL(fill_u8_kernel, dim3(1), dim3(1), 0, 0, t, (int64_t)1, (uint8_t)NODE_OUT);
         const int64_t n = (int64_t)b.nx * b.ny * b.nz;
         const unsigned g = (unsigned)std::min<int64_t>(nblk(n), 2048);
@@ -505,7 +506,15 @@ struct Geometry {
         hipFree(di);
     }
 
-    void build2(double time) {
+    // Grid2D::Prepare(time) into device cell arrays (cell type, v, T; the BC kind
+    // of every cell is no-slip); the caller frees them.
+    struct Cells2 {
+        uint8_t* cell = nullptr;
+        float *vx = nullptr, *vy = nullptr, *T = nullptr;
+        ~Cells2() { hipFree(cell); hipFree(vx); hipFree(vy); hipFree(T); }
+    };
+
+    void build2(double time, Cells2* keep = nullptr) {
         int fr;
         double sub;
         frame_at(time, fr, sub);
@@ -529,12 +538,18 @@ struct Geometry {
                     segs.push_back(Seg2{p[k], p[k + 1], vel[k], vel[k + 1], pass == 0 ? NODE_VALVE : NODE_BOUND});
             }
         const int64_t nc = (int64_t)cx * cy;
-        uint8_t* cell = dalloc<uint8_t>(nc);
-        float* cvx = dalloc<float>(nc);
-        float* cvy = dalloc<float>(nc);
-        float* cT = dalloc<float>(nc);
+        Cells2 own;
+        Cells2& C = keep ? *keep : own;
+        C.cell = dalloc<uint8_t>(nc);
+        C.vx = dalloc<float>(nc);
+        C.vy = dalloc<float>(nc);
+        C.T = dalloc<float>(nc);
+        uint8_t* cell = C.cell;
+        float* cvx = C.vx;
+        float* cvy = C.vy;
+        float* cT = C.T;
         unsigned long long* key = dalloc<unsigned long long>(nc);
-        Seg2* ds = dalloc<Seg2>(segs.size());
+        Seg2* ds = dalloc<Seg2>(std::max<size_t>(segs.size(), 1));
         try {
             CMCX_HIP(hipMemcpy(ds, segs.data(), sizeof(Seg2) * segs.size(), hipMemcpyHostToDevice));
             CMCX_HIP(hipMemset(cvx, 0, sizeof(float) * nc));
@@ -552,16 +567,17 @@ struct Geometry {
             flood(cell, Box{cx, cy, 1});
             hipLaunchKernelGGL(cell_data_kernel, dim3(nblk(nc)), dim3(256), 0, 0, cell, nc, cvx, cvy, cT,
                                (float)base_t);
-            hipLaunchKernelGGL(extrude_kernel, dim3(nblk(nc, 64)), dim3(64), 0, 0, cell, cvx, cvy, cT,
-                               Box{nx, ny, nz}, active_nz, depth_var, (float)base_t, type, bcv, bct, fv[0],
-                               fv[1], fv[2], fv[3]);
+            if (nz > 0)
+                hipLaunchKernelGGL(extrude_kernel, dim3(nblk(nc, 64)), dim3(64), 0, 0, cell, cvx, cvy, cT,
+                                   Box{nx, ny, nz}, active_nz, depth_var, (float)base_t, type, bcv, bct,
+                                   fv[0], fv[1], fv[2], fv[3]);
             CMCX_LAUNCH_CHECK();
             CMCX_HIP(hipDeviceSynchronize());
         } catch (...) {
-            hipFree(cell); hipFree(cvx); hipFree(cvy); hipFree(cT); hipFree(key); hipFree(ds);
+            hipFree(key); hipFree(ds);
             throw;
         }
-        hipFree(cell); hipFree(cvx); hipFree(cvy); hipFree(cT); hipFree(key); hipFree(ds);
+        hipFree(key); hipFree(ds);
     }
 };
 
@@ -735,8 +751,8 @@ int cmcx_geometry_load(int kind, const char* path, const double* spacing, double
             set_error("cmcx_geometry_load: grid spacings must be > 0");
             return CMCX_ERR_ARG;
         }
-    if (kind == CMCX_SHAPE2D && !(depth > 0)) {
-        set_error("cmcx_geometry_load: an extruded shape needs depth > 0");
+    if (kind == CMCX_SHAPE2D && depth < 0) {
+        set_error("cmcx_geometry_load: depth must be >= 0 (0: the 2D grid only)");
         return CMCX_ERR_ARG;
     }
     auto* h = new cmcx_geometry;
@@ -759,16 +775,20 @@ int cmcx_geometry_load(int kind, const char* path, const double* spacing, double
         if (align) { G.cx = align32(G.cx); G.cy = align32(G.cy); }
         G.nx = G.cx;
         G.ny = G.cy;
-        G.active_nz = (int)std::ceil(depth / spacing[2]) + 1;
-        G.nz = align ? align32(G.active_nz) : G.active_nz;
+        if (depth > 0) {
+            G.active_nz = (int)std::ceil(depth / spacing[2]) + 1;
+            G.nz = align ? align32(G.active_nz) : G.active_nz;
+        }
     }
-    if (G.nx < 1 || G.ny < 1 || G.nz < 1 || G.N() > ((int64_t)1 << 31) - 1) {
+    const bool only2d = G.kind == 2 && G.nz == 0;
+    if (G.nx < 1 || G.ny < 1 || (G.nz < 1 && !only2d) || G.N() > ((int64_t)1 << 31) - 1 ||
+        (int64_t)G.nx * G.ny > ((int64_t)1 << 31) - 1) {
         delete h;
         set_error("cmcx_geometry_load: grid empty or too large");
         return CMCX_ERR_TOO_LARGE;
     }
     try {
-        G.alloc_nodes();
+        if (!only2d) G.alloc_nodes();
     } catch (const HipError& e) {
         delete h;
         set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
@@ -800,6 +820,10 @@ int cmcx_geometry_build(cmcx_geometry* h, double time, uint8_t* type, uint8_t* b
                         double* vx, double* vy, double* vz, double* T) {
     if (!h) return CMCX_ERR_ARG;
     Geometry& G = h->g;
+    if (G.nz == 0) {
+        set_error("cmcx_geometry_build: loaded with depth 0 (the 2D grid only): use cmcx_geometry_build2d");
+        return CMCX_ERR_STATE;
+    }
     try {
         if (G.kind == 3) G.build3(time);
         else G.build2(time);
@@ -815,6 +839,36 @@ int cmcx_geometry_build(cmcx_geometry* h, double time, uint8_t* type, uint8_t* b
                 tmp.resize(n);
                 CMCX_HIP(hipMemcpy(tmp.data(), G.fv[f], sizeof(float) * n, hipMemcpyDeviceToHost));
                 for (int64_t q = 0; q < n; q++) dv[f][q] = (double)tmp[q];
+            }
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return CMCX_ERR_HIP;
+    }
+    return CMCX_OK;
+}
+
+int cmcx_geometry_build2d(cmcx_geometry* h, double time, uint8_t* type, uint8_t* bc, double* vx, double* vy,
+                          double* T) {
+    if (!h) return CMCX_ERR_ARG;
+    Geometry& G = h->g;
+    if (G.kind != 2) {
+        set_error("cmcx_geometry_build2d: a polyline (CMCX_SHAPE2D) input is needed");
+        return CMCX_ERR_ARG;
+    }
+    try {
+        Geometry::Cells2 C;
+        G.build2(time, &C);
+        const int64_t n = (int64_t)G.cx * G.cy;
+        if (type) CMCX_HIP(hipMemcpy(type, C.cell, n, hipMemcpyDeviceToHost));
+        if (bc) std::memset(bc, 0, n);  // BC_NOSLIP (Grid2D::RasterLine with bc_noslip)
+        const float* src[3] = {C.vx, C.vy, C.T};
+        double* dst[3] = {vx, vy, T};
+        std::vector<float> tmp;
+        for (int f = 0; f < 3; f++)
+            if (dst[f]) {
+                tmp.resize(n);
+                CMCX_HIP(hipMemcpy(tmp.data(), src[f], sizeof(float) * n, hipMemcpyDeviceToHost));
+                for (int64_t q = 0; q < n; q++) dst[f][q] = (double)tmp[q];
             }
     } catch (const HipError& e) {
         set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);

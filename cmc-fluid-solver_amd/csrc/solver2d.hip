@@ -241,6 +241,264 @@ __global__ void k2_div_finish(const double* __restrict__ part, int nb, double* _
     out[1] = c;
 }
 
+
+// ---- StableSolver2D (StableSolver2D.cpp:21-234): explicit advection-diffusion,
+// then a projection whose Poisson equation is relaxed cell by cell in place.
+
+// SolveU / SolveV (StableSolver2D.cpp:55-81) on NODE_IN cells: the state plus
+// dt times the advection-diffusion of the Picard layer `tl`, into w.
+template <class T>
+__global__ void k2s_solve_uv(int nx, int ny, const uint8_t* __restrict__ type, L3<T> cur, L3<T> tl,
+                             L3<T> w, T dt, T dx, T dy, T v_vis) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny || type[q] != NODE_IN) return;
+    const int64_t e = ny;
+    const T* U = tl.f[0];
+    const T* V = tl.f[1];
+    const T u = U[q], v = V[q];
+    const T ux = (U[q + e] - U[q - e]) / (2 * dx), uy = (U[q + 1] - U[q - 1]) / (2 * dy);
+    const T uxx = (U[q + e] - 2 * U[q] + U[q - e]) / (dx * dx);
+    const T uyy = (U[q + 1] - 2 * U[q] + U[q - 1]) / (dy * dy);
+    const T vx = (V[q + e] - V[q - e]) / (2 * dx), vy = (V[q + 1] - V[q - 1]) / (2 * dy);
+    const T vxx = (V[q + e] - 2 * V[q] + V[q - e]) / (dx * dx);
+    const T vyy = (V[q + 1] - 2 * V[q] + V[q - 1]) / (dy * dy);
+    w.f[0][q] = cur.f[0][q] + dt * (-u * ux - v * uy + v_vis * (uxx + uyy));
+    w.f[1][q] = cur.f[1][q] + dt * (-u * vx - v * vy + v_vis * (vxx + vyy));
+}
+
+// Project's divergence field (StableSolver2D.cpp:85-97): w.Ux + w.Vy on NODE_IN
+// cells, zero elsewhere.
+template <class T>
+__global__ void k2s_div(int nx, int ny, const uint8_t* __restrict__ type, L3<T> w, T dx, T dy,
+                        T* __restrict__ dv) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    if (type[q] != NODE_IN) {
+        dv[q] = 0;
+        return;
+    }
+    const T* U = w.f[0];
+    const T* V = w.f[1];
+    dv[q] = (U[q + ny] - U[q - ny]) / (2 * dx) + (V[q + 1] - V[q - 1]) / (2 * dy);
+}
+
+// Project's last loop (StableSolver2D.cpp:153-164): w minus the gradient of q
+// (ScalarField2D::grad, central differences in the field type) on NODE_IN cells.
+template <class T>
+__global__ void k2s_grad(int nx, int ny, const uint8_t* __restrict__ type, L3<T> w,
+                         const T* __restrict__ qf, T dx, T dy, L3<T> proj) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny || type[q] != NODE_IN) return;
+    const T gx = (qf[q + ny] - qf[q - ny]) / (2 * dx);
+    const T gy = (qf[q + 1] - qf[q - 1]) / (2 * dy);
+    proj.f[0][q] = w.f[0][q] - gx;
+    proj.f[1][q] = w.f[1][q] - gy;
+}
+
+// The relaxation's stopping test folds the cells' relative changes in
+// processing order with std::max(cur, err) = (cur < err) ? err : cur, from
+// err = 0: a NaN change (0/0) makes err NaN and the next cell's change
+// replaces it.  A sequence's fold as a function of the incoming err:
+//   kind 0 (empty): err unchanged;
+//   kind 1 (no NaN): max(err, m) (m when err is NaN);
+//   kind 2 (has a NaN): m, the fold since the last NaN (NaN if it was last).
+struct ErrFold {
+    int kind;
+    double m;
+};
+
+__device__ inline void fold_in(ErrFold& s, double x) {
+    if (x != x) {
+        s.kind = 2;
+        s.m = x;
+    } else if (s.kind == 0) {
+        s.kind = 1;
+        s.m = x;
+    } else {
+        s.m = (x < s.m) ? s.m : x;
+    }
+}
+
+// sequence a followed by sequence b
+__device__ inline ErrFold fold_cat(ErrFold a, ErrFold b) {
+    if (b.kind == 0) return a;
+    if (a.kind == 0 || b.kind == 2) return b;
+    ErrFold r;
+    r.kind = a.kind;
+    r.m = (b.m < a.m) ? a.m : b.m;
+    return r;
+}
+
+// Ordered fold over the work-group (thread order), result in thread 0.
+__device__ ErrFold fold_block(ErrFold s, ErrFold* sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        ErrFold o;
+        o.kind = __shfl_down(s.kind, off);
+        o.m = __shfl_down(s.m, off);
+        if (lane + off < 64) s = fold_cat(s, o);
+    }
+    if (lane == 0) sh[wave] = s;
+    __syncthreads();
+    ErrFold r{0, 0.0};
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nw; k++) r = fold_cat(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+
+// Cell flags of the relaxation: kind in bits 0-1 (0 outside, 1 NODE_IN, 2
+// NODE_BOUND or NODE_VALVE), then whether the neighbour (i-1, j), (i+1, j),
+// (i, j-1), (i, j+1) is NODE_IN (bits 2-5; cells beyond the grid are not).
+enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT = 32 };
+
+// Project's relaxation (StableSolver2D.cpp:99-151): sweeps over the boundary
+// cells, then the inner cells, each list in (i, j) order, each cell's new value
+// from its neighbours' current ones (those earlier in the order already
+// relaxed in this sweep), until the fold of the relative changes is below tol.
+//
+// One work-group, thread i owns row i (grids of up to 1024 rows).  Step k
+// relaxes the boundary cells on the anti-diagonal i + j = k and the inner
+// cells on i + j = k - 2: every value a cell reads is then the one the
+// sequential order gives it (boundary cells read inner neighbours from the
+// previous sweep and boundary neighbours from this one; inner cells read the
+// boundary cells of this sweep, inner neighbours before them from this sweep
+// and after them from the previous one), and the cells of one step touch no
+// cell another cell of the step reads, so one barrier per step orders
+// everything.  The fold runs per row in the row's own order, then over rows
+// in order: boundary rows, then inner rows.
+//
+// q lives per row over the row's span [lo, hi] of non-outside cells (outside
+// cells keep q = 0; so do cells beyond the grid: the reference reads outside
+// its arrays there, Grid2D.cpp:50-53 has no bounds check).  LDS holds q and
+// the flags (QLDS) or they stay in global memory.  At most max_sweeps sweeps
+// per launch; status = {sweeps so far, converged, last err (2 words)}; a
+// launch with resume = 1 continues from qspan.
+template <class T, bool QLDS>
+__global__ void __launch_bounds__(1024) k2s_poisson(
+    int nx, int ny, int S, const int32_t* __restrict__ spans, const uint8_t* __restrict__ fspan,
+    const T* __restrict__ dv, T* __restrict__ qspan, double dx2, double dy2, double rcp, double tol,
+    int max_sweeps, int resume, int32_t* __restrict__ status) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ ErrFold shb[16], shi[16];
+    __shared__ int s_go;
+    T* qs = QLDS ? reinterpret_cast<T*>(smem) : qspan;
+    const uint8_t* fl = QLDS ? smem + (size_t)S * sizeof(T) : fspan;
+    if (QLDS) {
+        for (int c = threadIdx.x; c < S; c += blockDim.x) {
+            qs[c] = resume ? qspan[c] : (T)0;
+            smem[(size_t)S * sizeof(T) + c] = fspan[c];
+        }
+    } else if (!resume) {
+        for (int c = threadIdx.x; c < S; c += blockDim.x) qs[c] = (T)0;
+    }
+    __syncthreads();
+    // spans (offset, lo, hi) of rows r - 1, r, r + 1
+    const int r = threadIdx.x;
+    const bool own = r < nx;
+    int off[3], lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int rd = r + d - 1;
+        const bool ok = own && rd >= 0 && rd < nx;
+        off[d] = ok ? spans[3 * rd] : 0;
+        lo[d] = ok ? spans[3 * rd + 1] : 0;
+        hi[d] = ok ? spans[3 * rd + 2] : -1;
+    }
+    auto qat = [&](int d, int j) -> double {
+        return (j >= lo[d] && j <= hi[d]) ? (double)qs[off[d] + j - lo[d]] : 0.0;
+    };
+    // The step's barrier: with q in LDS nothing global is shared between
+    // threads, so it waits for the LDS traffic only and the div loads of the
+    // next group stay in flight across it.
+    auto step_barrier = [&]() {
+        if (QLDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        else __syncthreads();
+    };
+    // div of the inner front's cells, loaded a group of GS steps ahead
+    constexpr int GS = 4;
+    double dcur[GS], dnext[GS];
+    auto load_group = [&](int kg, double (&d)[GS]) {
+#pragma unroll
+        for (int s = 0; s < GS; s++) {
+            const int j = kg + s - 2 - r;
+            d[s] = (own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
+        }
+    };
+    int sweeps = resume ? status[0] : 0;
+    const int steps = nx + ny + 1;
+    for (int n = 0; n < max_sweeps; n++) {
+        ErrFold fb{0, 0.0}, fi{0, 0.0};
+        load_group(0, dcur);
+        for (int kg = 0; kg < steps; kg += GS) {
+            load_group(kg + GS, dnext);
+#pragma unroll
+            for (int s = 0; s < GS; s++) {
+                const int k = kg + s;
+                if (k >= steps) break;  // uniform
+                const int jb = k - r, ji = k - 2 - r;
+                const bool vb = jb >= lo[1] && jb <= hi[1];
+                const bool vi = ji >= lo[1] && ji <= hi[1];
+                const int cb = off[1] + jb - lo[1], ci = off[1] + ji - lo[1];
+                const int fB = vb ? fl[cb] : 0;
+                const int fI = vi ? fl[ci] : 0;
+                // boundary front
+                if (fB & PF_BOUND) {
+                    const double up = qat(0, jb), dn = qat(2, jb);
+                    const double lf = qat(1, jb - 1), rt = qat(1, jb + 1);
+                    const double qo = (double)qs[cb];
+                    const double i0 = (fB & PF_UP) ? up : dn, i1 = (fB & PF_DOWN) ? dn : up;
+                    const double j0 = (fB & PF_LEFT) ? lf : rt, j1 = (fB & PF_RIGHT) ? rt : lf;
+                    // div is zero on boundary cells
+                    const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
+                    qs[cb] = (T)qn;
+                    fold_in(fb, fabs((qn - qo) / qn));
+                }
+                // inner front, two diagonals behind
+                if (fI & PF_IN) {
+                    const double i0 = qat(0, ji), i1 = qat(2, ji);
+                    const double j0 = qat(1, ji - 1), j1 = qat(1, ji + 1);
+                    const double qo = (double)qs[ci];
+                    const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - dcur[s] * dx2 * dy2);
+                    qs[ci] = (T)qn;
+                    fold_in(fi, fabs((qn - qo) / qn));
+                }
+                step_barrier();
+            }
+#pragma unroll
+            for (int s = 0; s < GS; s++) dcur[s] = dnext[s];
+        }
+        const ErrFold ab = fold_block(fb, shb);
+        const ErrFold ai = fold_block(fi, shi);
+        sweeps++;
+        if (threadIdx.x == 0) {
+            const ErrFold all = fold_cat(ab, ai);
+            const double err = all.kind == 0 ? 0.0 : all.m;  // kind 1: max(0, m) = m
+            const bool done = !(err >= tol);
+            s_go = done ? 0 : 1;
+            status[1] = done ? 1 : 0;
+            const unsigned long long b = (unsigned long long)__double_as_longlong(err);
+            status[2] = (int)(b & 0xffffffffu);
+            status[3] = (int)(b >> 32);
+        }
+        __syncthreads();
+        if (!s_go) break;
+    }
+    if (QLDS)
+        for (int c = threadIdx.x; c < S; c += blockDim.x) qspan[c] = qs[c];
+    if (threadIdx.x == 0) status[0] = sweeps;
+}
+
+// q over the whole grid (zero outside the spans) for the gradient.
+template <class T>
+__global__ void k2s_unspan(int nx, int ny, const int32_t* __restrict__ spans,
+                           const T* __restrict__ qspan, T* __restrict__ qf) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    const int i = (int)(q / ny), j = (int)(q % ny);
+    const int o = spans[3 * i], l = spans[3 * i + 1], h = spans[3 * i + 2];
+    qf[q] = (j >= l && j <= h) ? qspan[o + j - l] : (T)0;
+}
 }  // namespace
 
 struct ISolver2 {
@@ -248,6 +506,10 @@ struct ISolver2 {
     virtual int update_boundaries() = 0;
     virtual int step(double dt, int G, int L, double* err, int* iters) = 0;
     virtual int get_fields(void* u, void* v, void* p, void* T) = 0;
+    virtual int set_poisson(double tol, int64_t max_sweeps) = 0;
+    virtual int set_grid(const uint8_t* t, const uint8_t* bc, const double* vx, const double* vy,
+                         const double* T) = 0;
+    virtual int poisson_sweeps(int32_t* out, int max) = 0;
 };
 
 template <class T>
@@ -267,9 +529,23 @@ class Solver2D final : public ISolver2 {
     int64_t sc_half = 0;
     double* d_part = nullptr;
     double* d_err = nullptr;
+    int method = CMCX2D_ADI;
+    // Stable solver: next_w, div, q (whole grid and per-row spans), spans and
+    // cell flags of the relaxation, its status words
+    L3<T> nw{};
+    T *d_nw = nullptr, *d_dv = nullptr, *d_qf = nullptr, *d_qspan = nullptr;
+    int32_t* d_spans = nullptr;
+    uint8_t* d_fspan = nullptr;
+    int32_t* d_pstat = nullptr;
+    int S = 0;
+    double ptol = 1e-2;            // POISSON_ERR_THRESHOLD (StableSolver2D.h:23)
+    int64_t pcap = 1000000;        // the reference has no cap
+    std::vector<int32_t> sweeps;   // relaxation sweeps of each project of the last step
+    static constexpr int kPoisChunk = 2048;  // sweeps per launch
 
-    Solver2D(const cmcx2d_domain* d, const cmcx_params* p, hipStream_t stream) {
+    Solver2D(const cmcx2d_domain* d, const cmcx_params* p, int meth, hipStream_t stream) {
         st = stream;
+        method = meth;
         nx = d->dims[0];
         ny = d->dims[1];
         N = (int64_t)nx * ny;
@@ -295,10 +571,110 @@ class Solver2D final : public ISolver2 {
                 CMCX_HIP(hipStreamSynchronize(st));
             }
         }
-        build_systems(d->type);
+        if (method == CMCX2D_STABLE) build_relaxation(d->type);
+        else build_systems(d->type);
         CMCX_HIP(hipMalloc((void**)&d_part, sizeof(double) * 2 * kDiv2Blocks));
         CMCX_HIP(hipMalloc((void**)&d_err, sizeof(double) * 2));
+        if (method == CMCX2D_STABLE) {
+            // StableSolver2D::Init (StableSolver2D.cpp:44-52): next and temp from
+            // the state (copies that skip the last row and column)
+            copy(cur, next, kAll, 0);
+            copy(cur, temp, kAll, 0);
+        }
         CMCX_HIP(hipStreamSynchronize(st));
+    }
+
+    // PrepareIndices (StableSolver2D.cpp:167-188) for the fixed mask: per row
+    // the span of non-outside cells and the cells' relaxation flags.
+    void build_relaxation(const uint8_t* t) {
+        auto T_ = [&](int i, int j) -> int {
+            return (i < 0 || j < 0 || i >= nx || j >= ny) ? (int)NODE_OUT : t[(int64_t)i * ny + j];
+        };
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                if (T_(i, j) == NODE_IN && (i == 0 || j == 0 || i == nx - 1 || j == ny - 1))
+                    throw std::string("2D Stable: a NODE_IN cell on the grid's edge (" + std::to_string(i) +
+                                      ", " + std::to_string(j) + ")");
+        if (nx > 1024) throw std::string("2D Stable: grids of more than 1024 rows (dimx) are not supported");
+        std::vector<int32_t> sp(3 * (size_t)nx);
+        std::vector<uint8_t> fl;
+        for (int i = 0; i < nx; i++) {
+            int lo = 0, hi = -1;
+            for (int j = 0; j < ny; j++)
+                if (T_(i, j) != NODE_OUT) {
+                    if (hi < lo) lo = j;
+                    hi = j;
+                }
+            sp[3 * i] = (int32_t)fl.size();
+            sp[3 * i + 1] = lo;
+            sp[3 * i + 2] = hi;
+            for (int j = lo; j <= hi; j++) {
+                const int ty = T_(i, j);
+                int f = ty == NODE_IN ? PF_IN : (ty == NODE_BOUND || ty == NODE_VALVE) ? PF_BOUND : 0;
+                if (T_(i - 1, j) == NODE_IN) f |= PF_UP;
+                if (T_(i + 1, j) == NODE_IN) f |= PF_DOWN;
+                if (T_(i, j - 1) == NODE_IN) f |= PF_LEFT;
+                if (T_(i, j + 1) == NODE_IN) f |= PF_RIGHT;
+                fl.push_back((uint8_t)f);
+            }
+        }
+        // spans, flags and q per span (sized by the mask); the rest once
+        CMCX_HIP(hipStreamSynchronize(st));
+        hipFree(d_fspan);
+        hipFree(d_qspan);
+        d_fspan = nullptr;
+        d_qspan = nullptr;
+        S = (int)fl.size();
+        if (!d_spans) {
+            CMCX_HIP(hipMalloc((void**)&d_spans, sizeof(int32_t) * sp.size()));
+            CMCX_HIP(hipMalloc((void**)&d_nw, sizeof(T) * N * 2));
+            CMCX_HIP(hipMalloc((void**)&d_dv, sizeof(T) * N));
+            CMCX_HIP(hipMalloc((void**)&d_qf, sizeof(T) * N));
+            CMCX_HIP(hipMalloc((void**)&d_pstat, sizeof(int32_t) * 4));
+            CMCX_HIP(hipMemsetAsync(d_nw, 0, sizeof(T) * N * 2, st));
+            nw.f[0] = d_nw;
+            nw.f[1] = d_nw + N;
+            nw.f[2] = temp.f[2];  // never read or written (the step solves u and v only)
+        }
+        CMCX_HIP(hipMalloc((void**)&d_fspan, std::max(S, 1)));
+        CMCX_HIP(hipMalloc((void**)&d_qspan, sizeof(T) * std::max(S, 1)));
+        CMCX_HIP(hipMemcpyAsync(d_spans, sp.data(), sizeof(int32_t) * sp.size(), hipMemcpyHostToDevice, st));
+        if (S) CMCX_HIP(hipMemcpyAsync(d_fspan, fl.data(), S, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipStreamSynchronize(st));
+    }
+
+    // Grid2D::Prepare(t) between steps (FluidSolver2D.cpp:131): new cell types,
+    // BC kinds and node values; the line systems (ADI) or the relaxation's
+    // lists (Stable) follow the new types, as the reference rebuilds them
+    // every step.
+    int set_grid(const uint8_t* t, const uint8_t* bc, const double* vx, const double* vy,
+                 const double* Tt) override {
+        if (!t || !bc || !vx || !vy || !Tt) return CMCX_ERR_ARG;
+        for (int64_t i = 0; i < N; i++)
+            if (t[i] > 3 || bc[i] > 1) {
+                set_error("node type or BC kind out of range at node " + std::to_string(i));
+                return CMCX_ERR_ARG;
+            }
+        if (method == CMCX2D_STABLE) build_relaxation(t);
+        else {
+            CMCX_HIP(hipStreamSynchronize(st));
+            hipFree(d_rows);
+            hipFree(d_cols);
+            hipFree(scratch);
+            d_rows = d_cols = nullptr;
+            scratch = nullptr;
+            build_systems(t);
+        }
+        CMCX_HIP(hipMemcpyAsync(d_type, t, N, hipMemcpyHostToDevice, st));
+        CMCX_HIP(hipMemcpyAsync(d_bc, bc, N, hipMemcpyHostToDevice, st));
+        std::vector<T> h(N);
+        const double* src[3] = {vx, vy, Tt};
+        for (int f = 0; f < 3; f++) {
+            for (int64_t i = 0; i < N; i++) h[i] = (T)src[f][i];
+            CMCX_HIP(hipMemcpyAsync(g.f[f], h.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+            CMCX_HIP(hipStreamSynchronize(st));
+        }
+        return CMCX_OK;
     }
 
     ~Solver2D() override {
@@ -311,6 +687,13 @@ class Solver2D final : public ISolver2 {
         hipFree(scratch);
         hipFree(d_part);
         hipFree(d_err);
+        hipFree(d_nw);
+        hipFree(d_dv);
+        hipFree(d_qf);
+        hipFree(d_qspan);
+        hipFree(d_spans);
+        hipFree(d_fspan);
+        hipFree(d_pstat);
     }
 
     // AdiSolver2D::CreateSegments (the mask is fixed, so once).
@@ -412,8 +795,99 @@ class Solver2D final : public ISolver2 {
         return CMCX_OK;
     }
 
+    int set_poisson(double tol, int64_t max_sweeps) override {
+        if (!(tol > 0) || max_sweeps < 1) return CMCX_ERR_ARG;
+        ptol = tol;
+        pcap = max_sweeps;
+        return CMCX_OK;
+    }
+
+    int poisson_sweeps(int32_t* out, int max) override {
+        for (int k = 0; k < max && k < (int)sweeps.size(); k++) out[k] = sweeps[k];
+        return (int)sweeps.size();
+    }
+
+    template <bool QLDS>
+    void launch_poisson(int threads, size_t lds, double dx2, double dy2, double rcp, int chunk,
+                        int resume) {
+        hipLaunchKernelGGL((k2s_poisson<T, QLDS>), dim3(1), dim3(threads), lds, st, nx, ny, S,
+                           d_spans, d_fspan, d_dv, d_qspan, dx2, dy2, rcp, ptol, chunk, resume, d_pstat);
+        CMCX_LAUNCH_CHECK();
+    }
+
+    // StableSolver2D::Project (StableSolver2D.cpp:83-165): w -> next on NODE_IN.
+    // Returns false when the relaxation reached the sweep cap.
+    bool project() {
+        hipLaunchKernelGGL((k2s_div<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, d_type, nw,
+                           (T)prm.dx, (T)prm.dy, d_dv);
+        CMCX_LAUNCH_CHECK();
+        const double dx2 = prm.dx * prm.dx, dy2 = prm.dy * prm.dy, rcp = 0.5 / (dx2 + dy2);
+        const int threads = std::max(64, (nx + 63) / 64 * 64);
+        const size_t lds = (size_t)S * sizeof(T) + (size_t)S;
+        const bool in_lds = lds + 1024 <= 160 * 1024;
+        int64_t total = 0;
+        bool done = false;
+        for (int resume = 0; !done && total < pcap; resume = 1) {
+            const int chunk = (int)std::min<int64_t>(kPoisChunk, pcap - total);
+            if (in_lds) launch_poisson<true>(threads, lds, dx2, dy2, rcp, chunk, resume);
+            else launch_poisson<false>(threads, 0, dx2, dy2, rcp, chunk, resume);
+            int32_t h[4];
+            CMCX_HIP(hipMemcpyAsync(h, d_pstat, sizeof(h), hipMemcpyDeviceToHost, st));
+            CMCX_HIP(hipStreamSynchronize(st));
+            total = h[0];
+            done = h[1] != 0;
+        }
+        sweeps.push_back((int32_t)std::min<int64_t>(total, INT32_MAX));
+        if (!done) return false;
+        hipLaunchKernelGGL((k2s_unspan<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, d_spans, d_qspan,
+                           d_qf);
+        hipLaunchKernelGGL((k2s_grad<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, d_type, nw, d_qf,
+                           (T)prm.dx, (T)prm.dy, next);
+        CMCX_LAUNCH_CHECK();
+        return true;
+    }
+
+    // StableSolver2D::TimeStep (StableSolver2D.cpp:190-234)
+    int stable_step(double dt, int G, double* err_out, int* iters) {
+        if (G < 0) return CMCX_ERR_ARG;
+        prm.dt = dt;
+        sweeps.clear();
+        copy(cur, temp, kAll, 0);
+        double err = div_error(next);
+        int it;
+        for (it = 0; it < G || err > 0.1; it++) {
+            copy(cur, nw, kAll, 0);
+            hipLaunchKernelGGL((k2s_solve_uv<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, d_type, cur,
+                               temp, nw, (T)dt, (T)prm.dx, (T)prm.dy, (T)prm.v_vis);
+            CMCX_LAUNCH_CHECK();
+            if (!project()) {
+                if (iters) *iters = it + 1;
+                set_error("2D Stable: the Poisson relaxation did not converge within " +
+                          std::to_string(pcap) + " sweeps");
+                return CMCX_ERR_PICARD;
+            }
+            err = div_error(next);
+            copy(next, temp, kIn, 1);
+            if (it > 100 || err > 0.1 * 10) {
+                if (err_out) *err_out = err;
+                if (iters) *iters = it + 1;
+                set_error(it > 100 ? "2D: Picard iterations exceeded 100" : "2D: divergence error above 1.0");
+                return CMCX_ERR_PICARD;
+            }
+        }
+        hipLaunchKernelGGL((k2_clear_outer<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, N, d_type, next,
+                           (T)startT);
+        CMCX_LAUNCH_CHECK();
+        copy(next, cur, kAll, 0);
+        CMCX_HIP(hipStreamSynchronize(st));
+        if (err_out) *err_out = err;
+        if (iters) *iters = it;
+        return CMCX_OK;
+    }
+
     // AdiSolver2D::TimeStep (AdiSolver2D.cpp:279-323)
     int step(double dt, int G, int L, double* err_out, int* iters) override {
+        if (method == CMCX2D_STABLE) return stable_step(dt, G, err_out, iters);
         if (G < 0 || L < 1) return CMCX_ERR_ARG;
         prm.dt = dt;
         copy(cur, next, kAll, 0);
@@ -485,9 +959,18 @@ extern "C" {
 
 int cmcx2d_create(const cmcx2d_domain* d, const cmcx_params* p, int dtype, void* stream,
                   cmcx2d_solver** out) {
+    return cmcx2d_create_method(d, p, CMCX2D_ADI, dtype, stream, out);
+}
+
+int cmcx2d_create_method(const cmcx2d_domain* d, const cmcx_params* p, int method, int dtype,
+                         void* stream, cmcx2d_solver** out) {
     if (!d || !p || !out) return CMCX_ERR_ARG;
     *out = nullptr;
     if (dtype != CMCX_F32 && dtype != CMCX_F64) return CMCX_ERR_ARG;
+    if (method != CMCX2D_ADI && method != CMCX2D_STABLE) {
+        cmcx::set_error("cmcx2d_create_method: method must be CMCX2D_ADI or CMCX2D_STABLE");
+        return CMCX_ERR_ARG;
+    }
     if (d->dims[0] < 3 || d->dims[1] < 3 || !d->type || !d->bc || !d->v[0] || !d->v[1] || !d->T)
         return CMCX_ERR_ARG;
     const int64_t N = (int64_t)d->dims[0] * d->dims[1];
@@ -499,8 +982,9 @@ int cmcx2d_create(const cmcx2d_domain* d, const cmcx_params* p, int dtype, void*
     CMCX2_GUARD({
         auto* s = new cmcx2d_solver;
         try {
-            if (dtype == CMCX_F32) s->impl.reset(new cmcx::Solver2D<float>(d, p, (hipStream_t)stream));
-            else s->impl.reset(new cmcx::Solver2D<double>(d, p, (hipStream_t)stream));
+            if (dtype == CMCX_F32)
+                s->impl.reset(new cmcx::Solver2D<float>(d, p, method, (hipStream_t)stream));
+            else s->impl.reset(new cmcx::Solver2D<double>(d, p, method, (hipStream_t)stream));
         } catch (...) {
             delete s;
             throw;
@@ -523,6 +1007,25 @@ int cmcx2d_step(cmcx2d_solver* s, double dt, int G, int L, double* err, int* ite
 int cmcx2d_get_fields(cmcx2d_solver* s, void* u, void* v, void* p, void* T) {
     if (!s) return CMCX_ERR_ARG;
     CMCX2_GUARD(return s->impl->get_fields(u, v, p, T);)
+}
+
+int cmcx2d_set_grid(cmcx2d_solver* s, const uint8_t* type, const uint8_t* bc, const double* vx,
+                    const double* vy, const double* T) {
+    if (!s) return CMCX_ERR_ARG;
+    CMCX2_GUARD(return s->impl->set_grid(type, bc, vx, vy, T);)
+}
+
+int cmcx2d_set_poisson(cmcx2d_solver* s, double tol, int64_t max_sweeps) {
+    if (!s) return CMCX_ERR_ARG;
+    CMCX2_GUARD(return s->impl->set_poisson(tol, max_sweeps);)
+}
+
+int cmcx2d_poisson_sweeps(cmcx2d_solver* s, int32_t* out, int max, int* count) {
+    if (!s || !count || (max > 0 && !out)) return CMCX_ERR_ARG;
+    CMCX2_GUARD({
+        *count = s->impl->poisson_sweeps(out, max);
+        return CMCX_OK;
+    })
 }
 
 int cmcx2d_destroy(cmcx2d_solver* s) {

@@ -293,6 +293,13 @@ int cmcx_geometry_info(cmcx_geometry* g, int32_t* dims, double* bbox_min, double
    Prepare) into host arrays of dims[0]*dims[1]*dims[2] elements (any may be NULL). */
 int cmcx_geometry_build(cmcx_geometry* g, double time, uint8_t* type, uint8_t* bc_vel, uint8_t* bc_temp,
                         double* vx, double* vy, double* vz, double* T);
+/* Grid2D::Prepare(time) (src/FluidSolver2D/Grid2D.cpp:109-266, 456-480) of a
+   polyline input: the 2D solvers' grid, dims[0]*dims[1] cells, y fastest (cell
+   type, BC kind = BC_NOSLIP as with the reference's NoSlip setting, velocity,
+   temperature).  A CMCX_SHAPE2D load with depth = 0 makes only this grid
+   (dims[2] = 0; cmcx_geometry_build then fails). */
+int cmcx_geometry_build2d(cmcx_geometry* g, double time, uint8_t* type, uint8_t* bc, double* vx,
+                          double* vy, double* T);
 int cmcx_geometry_destroy(cmcx_geometry* g);
 
 /* ---- COARDS result files (the reference's netCDF output, IO.h:136-388) -----
@@ -332,6 +339,19 @@ typedef struct cmcx2d_domain {
 
 int cmcx2d_create(const cmcx2d_domain* dom, const cmcx_params* params, int dtype, void* stream,
                   cmcx2d_solver** out);
+/* The reference's 2D solver choice (Config.h:25, FluidSolver2D.cpp:65-70):
+   CMCX2D_ADI (AdiSolver2D; what cmcx2d_create makes) or CMCX2D_STABLE
+   (StableSolver2D.cpp:21-234: explicit advection-diffusion of u and v, then a
+   projection whose Poisson equation is relaxed cell by cell, boundary cells
+   then inner cells in (i, j) order, until the largest relative change of a
+   sweep is below 1e-2; T is carried, not solved).  The Stable solver needs
+   every NODE_IN cell off the grid's edge and at most 1024 rows (dimx); a
+   boundary cell on the edge reads q = 0 beyond the grid (the reference reads
+   outside its arrays there). */
+#define CMCX2D_ADI 1
+#define CMCX2D_STABLE 2
+int cmcx2d_create_method(const cmcx2d_domain* dom, const cmcx_params* params, int method, int dtype,
+                         void* stream, cmcx2d_solver** out);
 int cmcx2d_update_boundaries(cmcx2d_solver* s);
 /* Returns CMCX_ERR_PICARD where the reference exits (AdiSolver2D.cpp:303-313).
    iters: Picard iterations done (may be NULL). */
@@ -339,6 +359,18 @@ int cmcx2d_step(cmcx2d_solver* s, double dt, int num_global, int num_local, doub
                 int* iters);
 int cmcx2d_get_fields(cmcx2d_solver* s, void* u, void* v, void* p, void* T);
 int cmcx2d_destroy(cmcx2d_solver* s);
+/* Grid2D::Prepare(t) between steps (FluidSolver2D.cpp:131-133): new cell
+   types, BC kinds and node values for UpdateBoundaries and the next step (the
+   reference rebuilds its line lists every step, so the mask may change). */
+int cmcx2d_set_grid(cmcx2d_solver* s, const uint8_t* type, const uint8_t* bc, const double* vx,
+                    const double* vy, const double* T);
+/* Stable solver: relaxation tolerance (the reference's POISSON_ERR_THRESHOLD
+   1e-2, StableSolver2D.h:23) and a cap on sweeps per project (the reference has
+   none; default 1e6); a step that reaches the cap returns CMCX_ERR_PICARD. */
+int cmcx2d_set_poisson(cmcx2d_solver* s, double tol, int64_t max_sweeps);
+/* Stable solver: the relaxation sweeps of each project of the last step
+   (count = how many projects; up to max written to out). */
+int cmcx2d_poisson_sweeps(cmcx2d_solver* s, int32_t* out, int max, int* count);
 
 #ifdef __cplusplus
 }

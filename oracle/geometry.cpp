@@ -530,6 +530,9 @@ void* orc_geo_load(int kind, const char* path, double dx, double dy, double dz, 
             Grid& g = G.g;
             g.dx = dx; g.dy = dy; g.dz = dz; g.baseT = base_t;
             g.nx = G.nx; g.ny = G.ny;
+        }
+        if (ok && depth > 0) {  // depth <= 0: the 2D grid only (the 2D solvers' input)
+            Grid& g = G.g;
             G.active_nz = (int)ceil(depth / dz) + 1;
             g.nz = align ? align32(G.active_nz) : G.active_nz;
             g.alloc();
@@ -553,6 +556,18 @@ void orc_geo_build(void* p, double time, uint8_t* type, uint8_t* bcv, uint8_t* b
     for (size_t q = 0; q < g.type.size(); q++) {
         type[q] = g.type[q]; bcv[q] = g.bcv[q]; bct[q] = g.bct[q];
         vx[q] = g.vx[q]; vy[q] = g.vy[q]; vz[q] = g.vz[q]; T[q] = g.T[q];
+    }
+}
+
+// Grid2D::Prepare(time) of a Shape2D load, as the 2D solvers read it
+// (Grid2D::GetData): cell type, BC kind, velocity and temperature per cell.
+void orc_geo_build2d(void* p, double time, uint8_t* type, uint8_t* bc, double* vx, double* vy,
+                     double* T) {
+    Geo2& G = static_cast<OrcGeo*>(p)->g2;
+    build2d(G, time);
+    for (size_t q = 0; q < G.cur.size(); q++) {
+        type[q] = (uint8_t)G.cur[q].cell; bc[q] = (uint8_t)G.cur[q].type;
+        vx[q] = G.cur[q].vel.x; vy[q] = G.cur[q].vel.y; T[q] = G.cur[q].T;
     }
 }
 

@@ -14,6 +14,21 @@
 //   copy/merge/div    src/FluidSolver2D/TimeLayer2D.h:88-172
 //   thomas            src/Common/Algorithms.h:21-38
 //
+// The Stable solver (projection method, src/FluidSolver2D/StableSolver2D.cpp):
+//   stable_init       StableSolver2D.cpp:21-53 (next and temp copied from the state)
+//   prepare_indices   StableSolver2D.cpp:167-188
+//   solve_uv          StableSolver2D.cpp:55-81 (explicit advection-diffusion)
+//   project           StableSolver2D.cpp:83-165 (Poisson relaxation, gradient subtraction)
+//   stable_step       StableSolver2D.cpp:190-234
+//   grad              ScalarField2D.h:29-36
+// project's relaxation updates q in place, boundary list then inner list, each
+// in (i, j) order: a Gauss-Seidel sweep.  Its stopping test folds the cells'
+// relative changes with std::max(cur, err) = (cur < err) ? err : cur, so a 0/0
+// change (NaN) resets the running maximum and a NaN on the last cell ends the
+// loop.  Grid2D::GetType and the q fields have no bounds check
+// (Grid2D.cpp:50-53): a boundary cell on the grid's edge reads outside the
+// arrays.  Here the grid is taken as surrounded by NODE_OUT cells with q = 0.
+//
 // Layers the reference allocates with new[] and never fully writes (its copy
 // and merge loops stop at dimx-2 / dimy-2, TimeLayer2D.h:106-150) are zero
 // here: every layer, including the per-direction temp_local, starts zeroed.
@@ -51,6 +66,10 @@ struct Layer2 {
     T Uy(int i, int j) { return (U(i, j + 1) - U(i, j - 1)) / (2 * dy); }
     T Vx(int i, int j) { return (V(i + 1, j) - V(i - 1, j)) / (2 * dx); }
     T Vy(int i, int j) { return (V(i, j + 1) - V(i, j - 1)) / (2 * dy); }
+    T Uxx(int i, int j) { return (U(i + 1, j) - 2 * U(i, j) + U(i - 1, j)) / (dx * dx); }
+    T Uyy(int i, int j) { return (U(i, j + 1) - 2 * U(i, j) + U(i, j - 1)) / (dy * dy); }
+    T Vxx(int i, int j) { return (V(i + 1, j) - 2 * V(i, j) + V(i - 1, j)) / (dx * dx); }
+    T Vyy(int i, int j) { return (V(i, j + 1) - 2 * V(i, j) + V(i, j - 1)) / (dy * dy); }
     T Tx(int i, int j) { return (Tt(i + 1, j) - Tt(i - 1, j)) / (2 * dx); }
     T Ty(int i, int j) { return (Tt(i, j + 1) - Tt(i, j - 1)) / (2 * dy); }
     T DissFuncX(int i, int j) {
@@ -285,9 +304,127 @@ struct Solver2 {
         copy_all(next, cur);
         return 0;
     }
+
+    // ---- StableSolver2D -----------------------------------------------------
+    Layer2<T> next_w;
+    std::vector<T> q, dv;            // ScalarField2D q[0] and div
+    std::vector<int> inner, bound;   // flat indices, (i, j) order
+    std::vector<int> sweeps;         // relaxation sweeps of each project in the last step
+    double poisson_tol = 1e-2;       // POISSON_ERR_THRESHOLD (StableSolver2D.h:23)
+    int poisson_cap = 1 << 30;       // the reference has none
+
+    void stable_init() {
+        next_w.init(nx, ny, (T)gdx, (T)gdy);
+        q.assign((size_t)nx * ny, (T)0);
+        dv.assign((size_t)nx * ny, (T)0);
+        copy_all(cur, next);
+        copy_all(cur, temp);
+    }
+
+    void prepare_indices() {
+        inner.clear();
+        bound.clear();
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++) {
+                int t = GetType(i, j);
+                if (t == NODE_IN) inner.push_back((int)id(i, j));
+                else if (t == NODE_BOUND || t == NODE_VALVE) bound.push_back((int)id(i, j));
+            }
+    }
+
+    void solve_uv(T dt) {
+        Layer2<T>& tl = temp;
+        for (int c : inner) {
+            int i = c / ny, j = c % ny;
+            next_w.U(i, j) = cur.U(i, j) + dt * (-tl.U(i, j) * tl.Ux(i, j) - tl.V(i, j) * tl.Uy(i, j) +
+                                                 v_vis * (tl.Uxx(i, j) + tl.Uyy(i, j)));
+        }
+        for (int c : inner) {
+            int i = c / ny, j = c % ny;
+            next_w.V(i, j) = cur.V(i, j) + dt * (-tl.U(i, j) * tl.Vx(i, j) - tl.V(i, j) * tl.Vy(i, j) +
+                                                 v_vis * (tl.Vxx(i, j) + tl.Vyy(i, j)));
+        }
+    }
+
+    bool inside(int i, int j) const { return i >= 0 && j >= 0 && i < nx && j < ny; }
+    bool is_in(int i, int j) { return inside(i, j) && GetType(i, j) == NODE_IN; }
+    double qat(int i, int j) { return inside(i, j) ? (double)q[id(i, j)] : 0.0; }
+
+    void relax(int c, double i0, double i1, double j0, double j1, double dx2, double dy2, double rcp,
+               double& err) {
+        double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - (double)dv[c] * dx2 * dy2);
+        double ce = std::fabs((qn - (double)q[c]) / qn);
+        err = (ce < err) ? err : ce;  // std::max(ce, err)
+        q[c] = (T)qn;
+    }
+
+    // returns 1 when the sweep cap is reached
+    int project(Layer2<T>& w, Layer2<T>& proj) {
+        std::fill(dv.begin(), dv.end(), (T)0);
+        for (int c : inner) dv[c] = w.Ux(c / ny, c % ny) + w.Vy(c / ny, c % ny);
+        const double dx2 = gdx * gdx, dy2 = gdy * gdy, rcp = 0.5 / (dx2 + dy2);
+        std::fill(q.begin(), q.end(), (T)0);
+        int n = 0;
+        for (;;) {
+            double err = 0.0;
+            for (int c : bound) {
+                int i = c / ny, j = c % ny;
+                double i0 = is_in(i - 1, j) ? qat(i - 1, j) : qat(i + 1, j);
+                double i1 = is_in(i + 1, j) ? qat(i + 1, j) : qat(i - 1, j);
+                double j0 = is_in(i, j - 1) ? qat(i, j - 1) : qat(i, j + 1);
+                double j1 = is_in(i, j + 1) ? qat(i, j + 1) : qat(i, j - 1);
+                relax(c, i0, i1, j0, j1, dx2, dy2, rcp, err);
+            }
+            for (int c : inner) {
+                int i = c / ny, j = c % ny;
+                relax(c, qat(i - 1, j), qat(i + 1, j), qat(i, j - 1), qat(i, j + 1), dx2, dy2, rcp, err);
+            }
+            n++;
+            if (!(err >= poisson_tol)) break;
+            if (n >= poisson_cap) { sweeps.push_back(n); return 1; }
+        }
+        sweeps.push_back(n);
+        const T hx = (T)gdx, hy = (T)gdy;
+        for (int c : inner) {
+            int i = c / ny, j = c % ny;
+            T gx = (q[id(i + 1, j)] - q[id(i - 1, j)]) / (2 * hx);
+            T gy = (q[id(i, j + 1)] - q[id(i, j - 1)]) / (2 * hy);
+            proj.U(i, j) = w.U(i, j) - gx;
+            proj.V(i, j) = w.V(i, j) - gy;
+        }
+        return 0;
+    }
+
+    // 0 ok, 2 / 3 as time_step, 4 = relaxation sweep cap reached
+    int stable_step(T dt, int num_global, double* err_out, int* iters_out) {
+        copy_all(cur, temp);
+        prepare_indices();
+        sweeps.clear();
+        double err = div_error(next);
+        int it, status = 0;
+        for (it = 0; (it < num_global) || (err > 0.1); it++) {
+            copy_all(cur, next_w);
+            solve_uv(dt);
+            if (project(next_w, next)) { status = 4; break; }
+            err = div_error(next);
+            merge_type(next, temp, NODE_IN);
+            if (it > 100) { status = 2; break; }
+            if (err > 0.1 * 10) { status = 3; break; }
+        }
+        if (err_out) *err_out = err;
+        if (iters_out) *iters_out = it + (status ? 1 : 0);
+        if (status) return status;
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                if (GetType(i, j) == NODE_OUT) {
+                    next.U(i, j) = 0.0; next.V(i, j) = 0.0; next.Tt(i, j) = (T)startT;
+                }
+        copy_all(next, cur);
+        return 0;
+    }
 };
 
-struct H2 { int dtype; void* s; };
+struct H2 { int dtype; void* s; int method = 1; };
 
 }  // namespace orc2
 
@@ -329,7 +466,90 @@ void orc2_update_boundaries(void* h) { DISPATCH2(h, s->update_boundaries()); }
 
 int orc2_step(void* h, double dt, int G, int L, double* err, int* iters) {
     int r = 0;
-    DISPATCH2(h, r = s->time_step((decltype(s->v_T))dt, G, L, err, iters));
+    if (((H2*)h)->method == 2) DISPATCH2(h, r = s->stable_step((decltype(s->v_T))dt, G, err, iters));
+    else DISPATCH2(h, r = s->time_step((decltype(s->v_T))dt, G, L, err, iters));
+    return r;
+}
+
+// Grid2D::Prepare(t) between steps (FluidSolver2D.cpp:131): new cell types, BC
+// kinds and node values (UpdateBoundaries and the next step read them; both
+// solvers rebuild their lists from the types every step).
+void orc2_set_grid(void* h, const uint8_t* type, const uint8_t* bc, const double* vx, const double* vy,
+                   const double* T) {
+    DISPATCH2(h, {
+        using TT = decltype(s->v_T);
+        const size_t N = s->type.size();
+        s->type.assign(type, type + N);
+        s->bc.assign(bc, bc + N);
+        for (size_t i = 0; i < N; i++) {
+            s->gvx[i] = (TT)vx[i];
+            s->gvy[i] = (TT)vy[i];
+            s->gT[i] = (TT)T[i];
+        }
+    });
+}
+
+// SolveU / SolveV alone (test hook): out = cur + dt * (advection-diffusion of tl)
+// on NODE_IN cells; other cells of out as given.
+void orc2_stable_solve_uv(void* h, double dt, const double* cu, const double* cv, const double* tu,
+                          const double* tv, double* ou, double* ov) {
+    DISPATCH2(h, {
+        using TT = decltype(s->v_T);
+        const size_t N = s->type.size();
+        for (size_t i = 0; i < N; i++) {
+            s->cur.u[i] = (TT)cu[i]; s->cur.v[i] = (TT)cv[i];
+            s->temp.u[i] = (TT)tu[i]; s->temp.v[i] = (TT)tv[i];
+            s->next_w.u[i] = (TT)ou[i]; s->next_w.v[i] = (TT)ov[i];
+        }
+        s->prepare_indices();
+        s->solve_uv((TT)dt);
+        for (size_t i = 0; i < N; i++) { ou[i] = s->next_w.u[i]; ov[i] = s->next_w.v[i]; }
+    });
+}
+
+// Config.h solver ids: 1 = ADI, 2 = Stable (projection)
+void orc2_set_method(void* h, int method) {
+    ((H2*)h)->method = method;
+    if (method == 2) DISPATCH2(h, s->stable_init());
+}
+
+// relaxation settings of the Stable solver (the reference's: 1e-2, no cap)
+void orc2_set_poisson(void* h, double tol, int cap) {
+    DISPATCH2(h, { s->poisson_tol = tol; s->poisson_cap = cap; });
+}
+
+// relaxation sweeps of each project of the last step; returns their count
+int orc2_poisson_sweeps(void* h, int32_t* out, int max) {
+    int n = 0;
+    DISPATCH2(h, {
+        n = (int)s->sweeps.size();
+        for (int k = 0; k < n && k < max; k++) out[k] = s->sweeps[k];
+    });
+    return n;
+}
+
+// one project of the Stable solver on a given intermediate field w (all cells):
+// q, the projected u and v (NODE_IN cells; others as the layer held them) and
+// the sweep count.  Returns 1 at the sweep cap.
+int orc2_stable_project(void* h, const double* wu, const double* wv, double* q_out, double* pu,
+                        double* pv, int* sweeps) {
+    int r = 0;
+    DISPATCH2(h, {
+        using TT = decltype(s->v_T);
+        s->prepare_indices();
+        s->sweeps.clear();
+        for (size_t i = 0; i < s->next_w.u.size(); i++) {
+            s->next_w.u[i] = (TT)wu[i];
+            s->next_w.v[i] = (TT)wv[i];
+        }
+        r = s->project(s->next_w, s->next);
+        for (size_t i = 0; i < s->q.size(); i++) {
+            q_out[i] = s->q[i];
+            pu[i] = s->next.u[i];
+            pv[i] = s->next.v[i];
+        }
+        if (sweeps) *sweeps = s->sweeps.empty() ? 0 : s->sweeps.back();
+    });
     return r;
 }
 

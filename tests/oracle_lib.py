@@ -83,12 +83,22 @@ def load():
     L.orc2_get_layer_raw.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_d, _c_d, _c_d]
     L.orc2_segments.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_i32]
     L.orc2_segments.restype = ctypes.c_int
+    L.orc2_set_method.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc2_set_grid.argtypes = [ctypes.c_void_p, _c_u8, _c_u8, _c_d, _c_d, _c_d]
+    L.orc2_stable_solve_uv.argtypes = [ctypes.c_void_p, ctypes.c_double] + [_c_d] * 6
+    L.orc2_set_poisson.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.orc2_poisson_sweeps.argtypes = [ctypes.c_void_p, _c_i32, ctypes.c_int]
+    L.orc2_poisson_sweeps.restype = ctypes.c_int
+    L.orc2_stable_project.argtypes = [ctypes.c_void_p, _c_d, _c_d, _c_d, _c_d, _c_d,
+                                      ctypes.POINTER(ctypes.c_int)]
+    L.orc2_stable_project.restype = ctypes.c_int
     # geometry ingestion
     L.orc_geo_load.restype = ctypes.c_void_p
     L.orc_geo_load.argtypes = [ctypes.c_int, ctypes.c_char_p] + [ctypes.c_double] * 6 + [ctypes.c_int]
     L.orc_geo_dims.argtypes = [ctypes.c_void_p, _c_i32]
     L.orc_geo_build.argtypes = [ctypes.c_void_p, ctypes.c_double, _c_u8, _c_u8, _c_u8, _c_d, _c_d, _c_d, _c_d]
     L.orc_geo_grid2d.argtypes = [ctypes.c_void_p, _c_i32, _c_u8]
+    L.orc_geo_build2d.argtypes = [ctypes.c_void_p, ctypes.c_double, _c_u8, _c_u8, _c_d, _c_d, _c_d]
     L.orc_geo_free.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
@@ -228,7 +238,9 @@ class Oracle3D:
 
 
 class Oracle2D:
-    def __init__(self, dom, dtype: str = "f64", params=None):
+    """The 2D solvers: method "adi" (AdiSolver2D) or "stable" (StableSolver2D)."""
+
+    def __init__(self, dom, dtype: str = "f64", params=None, method: str = "adi"):
         L = load()
         self.L = L
         self.dom = dom
@@ -240,6 +252,43 @@ class Oracle2D:
         self.h = L.orc2_create(0 if dtype == "f32" else 1, dom.nx, dom.ny, dom.dx, dom.dy,
                                dom.base_t, *[_ptr(a, _c_u8) for a in arrs],
                                *[_ptr(a, _c_d) for a in vals], p.v_T, p.v_vis, p.t_vis, p.t_phi)
+        if method == "stable":
+            L.orc2_set_method(self.h, 2)
+
+    def set_grid(self, t, bc, vx, vy, T):
+        """Grid2D::Prepare(t) between steps: new cell types, BC kinds, node values."""
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in (t, bc)]
+        vals = [np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, T)]
+        self.L.orc2_set_grid(self.h, *[_ptr(a, _c_u8) for a in arrs], *[_ptr(a, _c_d) for a in vals])
+
+    def solve_uv(self, dt, cu, cv, tu, tv, ou=None, ov=None):
+        """SolveU / SolveV alone: (ou, ov) on NODE_IN cells from cur (cu, cv) and temp (tu, tv)."""
+        n = self.dom.nx * self.dom.ny
+        ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (cu, cv, tu, tv)]
+        ou = np.zeros(n) if ou is None else np.array(ou, dtype=np.float64)
+        ov = np.zeros(n) if ov is None else np.array(ov, dtype=np.float64)
+        self.L.orc2_stable_solve_uv(self.h, dt, *[_ptr(a, _c_d) for a in ins], _ptr(ou, _c_d), _ptr(ov, _c_d))
+        return ou, ov
+
+    def set_poisson(self, tol=1e-2, cap=1 << 30):
+        self.L.orc2_set_poisson(self.h, tol, cap)
+
+    def poisson_sweeps(self):
+        n = self.L.orc2_poisson_sweeps(self.h, ctypes.cast(None, _c_i32), 0)
+        out = np.zeros(max(n, 1), np.int32)
+        self.L.orc2_poisson_sweeps(self.h, _ptr(out, _c_i32), n)
+        return out[:n]
+
+    def project(self, wu, wv):
+        """One project of the Stable solver on (wu, wv): (q, pu, pv, sweeps, capped)."""
+        n = self.dom.nx * self.dom.ny
+        wu = np.ascontiguousarray(wu, dtype=np.float64)
+        wv = np.ascontiguousarray(wv, dtype=np.float64)
+        q, pu, pv = (np.zeros(n) for _ in range(3))
+        sw = ctypes.c_int(0)
+        r = self.L.orc2_stable_project(self.h, _ptr(wu, _c_d), _ptr(wv, _c_d), _ptr(q, _c_d),
+                                       _ptr(pu, _c_d), _ptr(pv, _c_d), ctypes.byref(sw))
+        return q, pu, pv, sw.value, r
 
     def __del__(self):
         try:
@@ -293,6 +342,18 @@ class OracleGeometry:
         self.L.orc_geo_build(self.h, time, *[_ptr(a, _c_u8) for a in (t, bv, bt)],
                              *[_ptr(a, _c_d) for a in (vx, vy, vz, T)])
         return t, bv, bt, vx, vy, vz, T
+
+    def build2d(self, time=0.0):
+        """Grid2D::Prepare(time): (type, bc, vx, vy, T) over the 2D grid, y fastest."""
+        d = np.zeros(2, np.int32)
+        self.L.orc_geo_grid2d(self.h, _ptr(d, _c_i32), ctypes.cast(None, _c_u8))
+        n = int(d[0] * d[1])
+        t, bc = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        vx, vy, T = (np.zeros(n) for _ in range(3))
+        self.L.orc_geo_build2d(self.h, time, _ptr(t, _c_u8), _ptr(bc, _c_u8), _ptr(vx, _c_d),
+                               _ptr(vy, _c_d), _ptr(T, _c_d))
+        self.dims2d = (int(d[0]), int(d[1]))
+        return t, bc, vx, vy, T
 
     def grid2d(self):
         d = np.zeros(2, np.int32)
