@@ -28,6 +28,9 @@ Prints one JSON line:
                        viscosity: backward/forward time and finite-difference check
   channel2d            the 2D ADI path on its 256 x 128 channel, GPU beside the
                        oracle on the host
+  stable2d             the 2D Stable (projection) path on the reference's box_pipe
+                       example: the Poisson relaxation's time per sweep and per
+                       wavefront step, GPU beside the oracle on the host
 """
 from __future__ import annotations
 
@@ -439,6 +442,66 @@ def channel2d_workload():
     }
 
 
+def stable2d_workload(steps: int = 2):
+    """The reference's 2D box_pipe example with its config (Stable solver,
+    120 x 135 cells, dx 0.7 mm, 30 steps per 0.035 s frame, num_global 2,
+    viscosity 0.05, density 1000, FP64): the first steps of its driver loop
+    (Prepare(t), UpdateBoundaries, TimeStep) on the GPU, grid built on the GPU,
+    and the same steps of the oracle on the host; states compared bit for bit.
+    The relaxation is bound by its chain of nx + ny + 1 barrier-separated
+    wavefront steps per sweep, not by HBM or MFMA."""
+    import oracle_lib
+    from cmcx.geometry import Grid2D
+    from cmcx.params import FluidParams
+    from cmcx.solver import StableSolver2D
+
+    path = os.path.join(ROOT, "tests", "golden", "geometry2d", "box_pipe_data.txt")
+    g = Grid2D(path, 0.0007, 0.0007, startT=1.0)
+    dt = g.cycle_length / (g.num_frames * 30)
+    d = g.Prepare(0.0)
+    p = FluidParams.dimensional(0.05, 1000.0, 287.0, 0.0257, 1000.0)
+    grids = [g.arrays((k + 1) * dt) for k in range(steps)]
+    s = StableSolver2D("f64").Init(d, p)
+    gpu_s, sweeps = 0.0, 0
+    for k in range(steps):
+        s.SetGrid(*grids[k])
+        s.UpdateBoundaries()
+        _sync()
+        t0 = time.perf_counter()
+        s.TimeStep(dt, 2, 1)
+        _sync()
+        gpu_s += time.perf_counter() - t0
+        sweeps += int(s.poisson_sweeps().sum())
+    o = oracle_lib.Oracle2D(d, "f64", params=p, method="stable")
+    cpu_s = 0.0
+    for k in range(steps):
+        o.set_grid(*grids[k])
+        o.update_boundaries()
+        t0 = time.perf_counter()
+        o.step(dt, 2, 1)
+        cpu_s += time.perf_counter() - t0
+    u, v, _, T = s.fields()
+    ou, ov, oT = o.layer(0)
+    same = bool(np.array_equal(u, ou) and np.array_equal(v, ov) and np.array_equal(T, oT))
+    wave = g.dimx + g.dimy + 1
+    s.close()
+    g.close()
+    return {
+        "workload": f"box_pipe 2D example {g.dimx}x{g.dimy}, Stable solver, dt {dt:.4g}, "
+                    f"first {steps} steps, G=2, fp64",
+        "relaxation_sweeps": sweeps,
+        "gpu_ms_per_step": gpu_s / steps * 1e3,
+        "gpu_us_per_sweep": gpu_s / sweeps * 1e6,
+        "wavefront_steps_per_sweep": wave,
+        "gpu_ns_per_wavefront_step": gpu_s / sweeps / wave * 1e9,
+        "cpu_ms_per_step": cpu_s / steps * 1e3,
+        "cpu_us_per_sweep": cpu_s / sweeps * 1e6,
+        "cpu_kind": "port (oracle, 1 thread; the reference's relaxation is serial)",
+        "bitwise_equal_to_oracle": same,
+        "bound": "dependency chain: nx + ny + 1 barrier-separated steps per sweep in one work-group",
+    }
+
+
 def measured_traffic(size: int, path: str):
     """HBM bytes per sweep launch from the committed PMC summary of this
     workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a plain copy), or None."""
@@ -615,6 +678,7 @@ def main():
     batch = None if args.no_batch else time_batch(max(4, args.steps // 2), 2)
     adjoint = None if args.no_adjoint else adjoint_workload(ndir=args.adjoint_dirs)
     two_d = None if args.no_2d else channel2d_workload()
+    stable = None if args.no_2d else stable2d_workload()
     cpu = None if args.no_cpu else cpu_baseline(dom, args.cpu_budget)
     parity = (None if args.no_parity or head_path != "fused"
               else headline_parity(dom, head["variants"]))
@@ -684,6 +748,8 @@ def main():
         line["adjoint"] = adjoint
     if two_d is not None:
         line["channel2d"] = two_d
+    if stable is not None:
+        line["stable2d"] = stable
     if parity is not None:
         line["parity"] = parity
     if plain is not None:

@@ -295,27 +295,40 @@ __global__ void k2s_grad(int nx, int ny, const uint8_t* __restrict__ type, L3<T>
     proj.f[1][q] = w.f[1][q] - gy;
 }
 
-// The relaxation's stopping test folds the cells' relative changes in
-// processing order with std::max(cur, err) = (cur < err) ? err : cur, from
-// err = 0: a NaN change (0/0) makes err NaN and the next cell's change
-// replaces it.  A sequence's fold as a function of the incoming err:
-//   kind 0 (empty): err unchanged;
-//   kind 1 (no NaN): max(err, m) (m when err is NaN);
-//   kind 2 (has a NaN): m, the fold since the last NaN (NaN if it was last).
+// The relaxation's stopping test folds the cells' relative changes
+// |(qn - qo) / qn| in sweep order with std::max(cur, err) from err = 0, then
+// continues while err >= tol: a NaN change (0/0) makes err NaN and the next
+// cell's change replaces it, so the test depends only on the changes after
+// the last NaN (none of them >= tol, or a NaN last, ends the loop).  A
+// sequence's summary: kind 0 (empty), 1 (no NaN), 2 (a NaN, then ge counts
+// only the changes after the last one); ge: some change >= tol.
 struct ErrFold {
     int kind;
-    double m;
+    int ge;
 };
 
-__device__ inline void fold_in(ErrFold& s, double x) {
-    if (x != x) {
+// One cell's change into the summary.  |qn - qo| against tol |qn| decides
+// without dividing away from the threshold (the margin covers the rounding of
+// both sides); close to it, and for zero or non-finite values, the change is
+// divided out as the reference does, so the decision is the reference's.
+__device__ inline void fold_cell(ErrFold& s, double qn, double qo, double tol) {
+    const double d = qn - qo;
+    const double a = fabs(d), m = fabs(qn), b = tol * m;
+    const bool plain = m >= 1e-290 && m <= 1.79e308 && fabs(qo) <= 1.79e308 && (b > 1e-300 || tol == 0.0);
+    bool nan = false, ge;
+    if (plain && a < b * (1.0 - 1e-12)) ge = false;
+    else if (plain && a > b * (1.0 + 1e-12)) ge = true;
+    else {
+        const double ce = fabs(d / qn);
+        nan = ce != ce;
+        ge = ce >= tol;
+    }
+    if (nan) {
         s.kind = 2;
-        s.m = x;
-    } else if (s.kind == 0) {
-        s.kind = 1;
-        s.m = x;
+        s.ge = 0;
     } else {
-        s.m = (x < s.m) ? s.m : x;
+        if (s.kind == 0) s.kind = 1;
+        s.ge |= ge ? 1 : 0;
     }
 }
 
@@ -323,10 +336,7 @@ __device__ inline void fold_in(ErrFold& s, double x) {
 __device__ inline ErrFold fold_cat(ErrFold a, ErrFold b) {
     if (b.kind == 0) return a;
     if (a.kind == 0 || b.kind == 2) return b;
-    ErrFold r;
-    r.kind = a.kind;
-    r.m = (b.m < a.m) ? a.m : b.m;
-    return r;
+    return ErrFold{a.kind, a.ge | b.ge};
 }
 
 // Ordered fold over the work-group (thread order), result in thread 0.
@@ -335,12 +345,12 @@ __device__ ErrFold fold_block(ErrFold s, ErrFold* sh) {
     for (int off = 1; off < 64; off <<= 1) {
         ErrFold o;
         o.kind = __shfl_down(s.kind, off);
-        o.m = __shfl_down(s.m, off);
+        o.ge = __shfl_down(s.ge, off);
         if (lane + off < 64) s = fold_cat(s, o);
     }
     if (lane == 0) sh[wave] = s;
     __syncthreads();
-    ErrFold r{0, 0.0};
+    ErrFold r{0, 0};
     if (threadIdx.x == 0)
         for (int k = 0; k < nw; k++) r = fold_cat(r, sh[k]);
     __syncthreads();
@@ -355,32 +365,35 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // Project's relaxation (StableSolver2D.cpp:99-151): sweeps over the boundary
 // cells, then the inner cells, each list in (i, j) order, each cell's new value
 // from its neighbours' current ones (those earlier in the order already
-// relaxed in this sweep), until the fold of the relative changes is below tol.
+// relaxed in this sweep), until the stopping test passes.
 //
-// One work-group, thread i owns row i (grids of up to 1024 rows).  Step k
-// relaxes the boundary cells on the anti-diagonal i + j = k and the inner
-// cells on i + j = k - 2: every value a cell reads is then the one the
-// sequential order gives it (boundary cells read inner neighbours from the
-// previous sweep and boundary neighbours from this one; inner cells read the
-// boundary cells of this sweep, inner neighbours before them from this sweep
-// and after them from the previous one), and the cells of one step touch no
-// cell another cell of the step reads, so one barrier per step orders
-// everything.  The fold runs per row in the row's own order, then over rows
-// in order: boundary rows, then inner rows.
+// One work-group of two thread sets of rp threads (rp >= nx, a multiple of
+// 64, grids of up to 512 rows): thread r of the first set relaxes the
+// boundary cell (r, k - r) at step k, thread r of the second the inner cell
+// (r, k - 2 - r).  Every value a cell reads is then the one the sequential
+// order gives it (boundary cells read inner neighbours from the previous sweep
+// and boundary neighbours from this one; inner cells read the boundary cells
+// of this sweep, inner neighbours before them from this sweep and after them
+// from the previous one), and the cells of one step touch no cell another cell
+// of the step reads, so one barrier per step orders everything.  A step's six
+// reads (the cell, its row neighbours, its flags, the two neighbour rows) go
+// out together.  The fold runs per row in the row's own order, then over the
+// threads in order: boundary rows, then inner rows.
 //
-// q lives per row over the row's span [lo, hi] of non-outside cells (outside
-// cells keep q = 0; so do cells beyond the grid: the reference reads outside
-// its arrays there, Grid2D.cpp:50-53 has no bounds check).  LDS holds q and
-// the flags (QLDS) or they stay in global memory.  At most max_sweeps sweeps
-// per launch; status = {sweeps so far, converged, last err (2 words)}; a
-// launch with resume = 1 continues from qspan.
+// q lives per row over the row's span [lo, hi] of non-outside cells plus a
+// zero guard cell at each end (row r at spans[3r]: column j at
+// spans[3r] + 1 + j - lo); outside cells keep q = 0, and so do cells beyond the
+// grid (the reference reads outside its arrays there, Grid2D.cpp:50-53 has no
+// bounds check).  LDS holds q and the flags (QLDS) or they stay in global
+// memory.  At most max_sweeps sweeps per launch; status = {sweeps so far,
+// converged}; a launch with resume = 1 continues from qspan.
 template <class T, bool QLDS>
 __global__ void __launch_bounds__(1024) k2s_poisson(
-    int nx, int ny, int S, const int32_t* __restrict__ spans, const uint8_t* __restrict__ fspan,
+    int nx, int ny, int S, int rp, const int32_t* __restrict__ spans, const uint8_t* __restrict__ fspan,
     const T* __restrict__ dv, T* __restrict__ qspan, double dx2, double dy2, double rcp, double tol,
     int max_sweeps, int resume, int32_t* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ErrFold shb[16], shi[16];
+    __shared__ ErrFold sh[16];
     __shared__ int s_go;
     T* qs = QLDS ? reinterpret_cast<T*>(smem) : qspan;
     const uint8_t* fl = QLDS ? smem + (size_t)S * sizeof(T) : fspan;
@@ -393,20 +406,27 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
         for (int c = threadIdx.x; c < S; c += blockDim.x) qs[c] = (T)0;
     }
     __syncthreads();
-    // spans (offset, lo, hi) of rows r - 1, r, r + 1
-    const int r = threadIdx.x;
+    const int inner = (int)threadIdx.x >= rp ? 1 : 0;  // uniform per wave
+    const int r = (int)threadIdx.x - inner * rp;
     const bool own = r < nx;
-    int off[3], lo[3], hi[3];
+    // padded spans of rows r - 1, r, r + 1: storage [a, b] holds columns
+    // lo - 1 .. hi + 1 (column j at base + j); a cell is inside for lo <= j <= hi
+    int base[3], lo[3], hi[3], a[3], b[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int rd = r + d - 1;
         const bool ok = own && rd >= 0 && rd < nx;
-        off[d] = ok ? spans[3 * rd] : 0;
+        a[d] = ok ? spans[3 * rd] : 0;
         lo[d] = ok ? spans[3 * rd + 1] : 0;
         hi[d] = ok ? spans[3 * rd + 2] : -1;
+        base[d] = a[d] + 1 - lo[d];
+        b[d] = a[d] + (hi[d] - lo[d] + 1) + 1;
     }
-    auto qat = [&](int d, int j) -> double {
-        return (j >= lo[d] && j <= hi[d]) ? (double)qs[off[d] + j - lo[d]] : 0.0;
+    auto clampi = [](int x, int a, int b) { return min(max(x, a), b); };
+    auto inside = [&](int d, int j) -> bool { return j >= lo[d] && j <= hi[d]; };
+    auto mask = [](T v, bool in) -> double {
+        const long long m = in ? -1ll : 0ll;
+        return __longlong_as_double(__double_as_longlong((double)v) & m);
     };
     // The step's barrier: with q in LDS nothing global is shared between
     // threads, so it waits for the LDS traffic only and the div loads of the
@@ -415,71 +435,68 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
         if (QLDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         else __syncthreads();
     };
-    // div of the inner front's cells, loaded a group of GS steps ahead
+    // div of the inner set's cells, loaded a group of GS steps ahead
     constexpr int GS = 4;
     double dcur[GS], dnext[GS];
     auto load_group = [&](int kg, double (&d)[GS]) {
 #pragma unroll
         for (int s = 0; s < GS; s++) {
             const int j = kg + s - 2 - r;
-            d[s] = (own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
+            d[s] = (inner && own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
         }
     };
     int sweeps = resume ? status[0] : 0;
     const int steps = nx + ny + 1;
     for (int n = 0; n < max_sweeps; n++) {
-        ErrFold fb{0, 0.0}, fi{0, 0.0};
+        ErrFold fo{0, 0};
         load_group(0, dcur);
         for (int kg = 0; kg < steps; kg += GS) {
             load_group(kg + GS, dnext);
 #pragma unroll
             for (int s = 0; s < GS; s++) {
-                const int k = kg + s;
-                if (k >= steps) break;  // uniform
-                const int jb = k - r, ji = k - 2 - r;
-                const bool vb = jb >= lo[1] && jb <= hi[1];
-                const bool vi = ji >= lo[1] && ji <= hi[1];
-                const int cb = off[1] + jb - lo[1], ci = off[1] + ji - lo[1];
-                const int fB = vb ? fl[cb] : 0;
-                const int fI = vi ? fl[ci] : 0;
-                // boundary front
-                if (fB & PF_BOUND) {
-                    const double up = qat(0, jb), dn = qat(2, jb);
-                    const double lf = qat(1, jb - 1), rt = qat(1, jb + 1);
-                    const double qo = (double)qs[cb];
-                    const double i0 = (fB & PF_UP) ? up : dn, i1 = (fB & PF_DOWN) ? dn : up;
-                    const double j0 = (fB & PF_LEFT) ? lf : rt, j1 = (fB & PF_RIGHT) ? rt : lf;
-                    // div is zero on boundary cells
-                    const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
-                    qs[cb] = (T)qn;
-                    fold_in(fb, fabs((qn - qo) / qn));
-                }
-                // inner front, two diagonals behind
-                if (fI & PF_IN) {
-                    const double i0 = qat(0, ji), i1 = qat(2, ji);
-                    const double j0 = qat(1, ji - 1), j1 = qat(1, ji + 1);
-                    const double qo = (double)qs[ci];
-                    const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - dcur[s] * dx2 * dy2);
-                    qs[ci] = (T)qn;
-                    fold_in(fi, fabs((qn - qo) / qn));
+                if (kg + s >= steps) break;  // uniform
+                const int j = kg + s - r - 2 * inner;
+                // own row: the cell and its two row neighbours (clamped so the
+                // three reads stay in the row's storage), its flags, and the
+                // neighbour rows at column j, issued together; the empty asm
+                // takes them all, so none is sunk into a branch of its own
+                const int c = clampi(base[1] + j, a[1] + 1, max(b[1] - 1, a[1] + 1));
+                const int ca = clampi(base[0] + j, a[0], b[0]);
+                const int cb = clampi(base[2] + j, a[2], b[2]);
+                T lq = qs[c], ll = qs[c - 1], lr = qs[c + 1], la = qs[ca], lb = qs[cb];
+                int lf = fl[c];
+                asm volatile("" : "+v"(lq), "+v"(ll), "+v"(lr), "+v"(la), "+v"(lb), "+v"(lf));
+                const bool in1 = inside(1, j);
+                const int f = lf & (in1 ? 0xff : 0);
+                const double qo = (double)lq, lf_ = (double)ll, rt = (double)lr;
+                const double up = mask(la, inside(0, j)), dn = mask(lb, inside(2, j));
+                if (!inner) {
+                    if (f & PF_BOUND) {
+                        const double i0 = (f & PF_UP) ? up : dn, i1 = (f & PF_DOWN) ? dn : up;
+                        const double j0 = (f & PF_LEFT) ? lf_ : rt, j1 = (f & PF_RIGHT) ? rt : lf_;
+                        // div is zero on boundary cells
+                        const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
+                        qs[c] = (T)qn;
+                        fold_cell(fo, qn, qo, tol);
+                    }
+                } else {
+                    if (f & PF_IN) {
+                        const double qn = rcp * ((up + dn) * dy2 + (lf_ + rt) * dx2 - dcur[s] * dx2 * dy2);
+                        qs[c] = (T)qn;
+                        fold_cell(fo, qn, qo, tol);
+                    }
                 }
                 step_barrier();
             }
 #pragma unroll
             for (int s = 0; s < GS; s++) dcur[s] = dnext[s];
         }
-        const ErrFold ab = fold_block(fb, shb);
-        const ErrFold ai = fold_block(fi, shi);
+        const ErrFold all = fold_block(fo, sh);
         sweeps++;
         if (threadIdx.x == 0) {
-            const ErrFold all = fold_cat(ab, ai);
-            const double err = all.kind == 0 ? 0.0 : all.m;  // kind 1: max(0, m) = m
-            const bool done = !(err >= tol);
+            const bool done = !(all.kind != 0 && all.ge);
             s_go = done ? 0 : 1;
             status[1] = done ? 1 : 0;
-            const unsigned long long b = (unsigned long long)__double_as_longlong(err);
-            status[2] = (int)(b & 0xffffffffu);
-            status[3] = (int)(b >> 32);
         }
         __syncthreads();
         if (!s_go) break;
@@ -497,7 +514,7 @@ __global__ void k2s_unspan(int nx, int ny, const int32_t* __restrict__ spans,
     if (q >= (int64_t)nx * ny) return;
     const int i = (int)(q / ny), j = (int)(q % ny);
     const int o = spans[3 * i], l = spans[3 * i + 1], h = spans[3 * i + 2];
-    qf[q] = (j >= l && j <= h) ? qspan[o + j - l] : (T)0;
+    qf[q] = (j >= l && j <= h) ? qspan[o + 1 + j - l] : (T)0;
 }
 }  // namespace
 
@@ -595,7 +612,7 @@ class Solver2D final : public ISolver2 {
                 if (T_(i, j) == NODE_IN && (i == 0 || j == 0 || i == nx - 1 || j == ny - 1))
                     throw std::string("2D Stable: a NODE_IN cell on the grid's edge (" + std::to_string(i) +
                                       ", " + std::to_string(j) + ")");
-        if (nx > 1024) throw std::string("2D Stable: grids of more than 1024 rows (dimx) are not supported");
+        if (nx > 512) throw std::string("2D Stable: grids of more than 512 rows (dimx) are not supported");
         std::vector<int32_t> sp(3 * (size_t)nx);
         std::vector<uint8_t> fl;
         for (int i = 0; i < nx; i++) {
@@ -605,9 +622,11 @@ class Solver2D final : public ISolver2 {
                     if (hi < lo) lo = j;
                     hi = j;
                 }
+            // storage: a zero guard cell, columns lo .. hi, a zero guard cell
             sp[3 * i] = (int32_t)fl.size();
             sp[3 * i + 1] = lo;
             sp[3 * i + 2] = hi;
+            fl.push_back(0);
             for (int j = lo; j <= hi; j++) {
                 const int ty = T_(i, j);
                 int f = ty == NODE_IN ? PF_IN : (ty == NODE_BOUND || ty == NODE_VALVE) ? PF_BOUND : 0;
@@ -617,7 +636,9 @@ class Solver2D final : public ISolver2 {
                 if (T_(i, j + 1) == NODE_IN) f |= PF_RIGHT;
                 fl.push_back((uint8_t)f);
             }
+            fl.push_back(0);
         }
+        fl.push_back(0);  // one more guard: the last row's reads stay inside
         // spans, flags and q per span (sized by the mask); the rest once
         CMCX_HIP(hipStreamSynchronize(st));
         hipFree(d_fspan);
@@ -808,9 +829,8 @@ class Solver2D final : public ISolver2 {
     }
 
     template <bool QLDS>
-    void launch_poisson(int threads, size_t lds, double dx2, double dy2, double rcp, int chunk,
-                        int resume) {
-        hipLaunchKernelGGL((k2s_poisson<T, QLDS>), dim3(1), dim3(threads), lds, st, nx, ny, S,
+    void launch_poisson(int rp, size_t lds, double dx2, double dy2, double rcp, int chunk, int resume) {
+        hipLaunchKernelGGL((k2s_poisson<T, QLDS>), dim3(1), dim3(2 * rp), lds, st, nx, ny, S, rp,
                            d_spans, d_fspan, d_dv, d_qspan, dx2, dy2, rcp, ptol, chunk, resume, d_pstat);
         CMCX_LAUNCH_CHECK();
     }
@@ -822,16 +842,16 @@ class Solver2D final : public ISolver2 {
                            (T)prm.dx, (T)prm.dy, d_dv);
         CMCX_LAUNCH_CHECK();
         const double dx2 = prm.dx * prm.dx, dy2 = prm.dy * prm.dy, rcp = 0.5 / (dx2 + dy2);
-        const int threads = std::max(64, (nx + 63) / 64 * 64);
+        const int rp = std::max(64, (nx + 63) / 64 * 64);  // threads per set
         const size_t lds = (size_t)S * sizeof(T) + (size_t)S;
         const bool in_lds = lds + 1024 <= 160 * 1024;
         int64_t total = 0;
         bool done = false;
         for (int resume = 0; !done && total < pcap; resume = 1) {
             const int chunk = (int)std::min<int64_t>(kPoisChunk, pcap - total);
-            if (in_lds) launch_poisson<true>(threads, lds, dx2, dy2, rcp, chunk, resume);
-            else launch_poisson<false>(threads, 0, dx2, dy2, rcp, chunk, resume);
-            int32_t h[4];
+            if (in_lds) launch_poisson<true>(rp, lds, dx2, dy2, rcp, chunk, resume);
+            else launch_poisson<false>(rp, 0, dx2, dy2, rcp, chunk, resume);
+            int32_t h[2];
             CMCX_HIP(hipMemcpyAsync(h, d_pstat, sizeof(h), hipMemcpyDeviceToHost, st));
             CMCX_HIP(hipStreamSynchronize(st));
             total = h[0];

@@ -345,7 +345,7 @@ int cmcx2d_create(const cmcx2d_domain* dom, const cmcx_params* params, int dtype
    projection whose Poisson equation is relaxed cell by cell, boundary cells
    then inner cells in (i, j) order, until the largest relative change of a
    sweep is below 1e-2; T is carried, not solved).  The Stable solver needs
-   every NODE_IN cell off the grid's edge and at most 1024 rows (dimx); a
+   every NODE_IN cell off the grid's edge and at most 512 rows (dimx); a
    boundary cell on the edge reads q = 0 beyond the grid (the reference reads
    outside its arrays there). */
 #define CMCX2D_ADI 1
