@@ -498,7 +498,7 @@ def stable2d_workload(steps: int = 2):
         "cpu_us_per_sweep": cpu_s / sweeps * 1e6,
         "cpu_kind": "port (oracle, 1 thread; the reference's relaxation is serial)",
         "bitwise_equal_to_oracle": same,
-        "bound": "dependency chain: nx + ny + 1 barrier-separated steps per sweep in one work-group",
+        "bound": "nx + ny + 1 barrier-separated wavefront steps per sweep, sweeps in groups of up to 8 in one work-group",
     }
 
 

@@ -368,17 +368,29 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // relaxed in this sweep), until the stopping test passes.
 //
 // One work-group of two thread sets of rp threads (rp >= nx, a multiple of
-// 64, grids of up to 512 rows): thread r of the first set relaxes the
-// boundary cell (r, k - r) at step k, thread r of the second the inner cell
-// (r, k - 2 - r).  Every value a cell reads is then the one the sequential
-// order gives it (boundary cells read inner neighbours from the previous sweep
-// and boundary neighbours from this one; inner cells read the boundary cells
-// of this sweep, inner neighbours before them from this sweep and after them
-// from the previous one), and the cells of one step touch no cell another cell
-// of the step reads, so one barrier per step orders everything.  A step's six
-// reads (the cell, its row neighbours, its flags, the two neighbour rows) go
-// out together.  The fold runs per row in the row's own order, then over the
-// threads in order: boundary rows, then inner rows.
+// 64, grids of up to 512 rows): thread r of the first set relaxes row r's
+// boundary cells, thread r of the second its inner cells.  A sweep's step k
+// relaxes the boundary cell (r, k - r) and the inner cell (r, k - 2 - r).
+// Every value a cell reads is then the one the sequential order gives it
+// (boundary cells read inner neighbours from the previous sweep and boundary
+// neighbours from this one; inner cells read the boundary cells of this sweep,
+// inner neighbours before them from this sweep and after them from the
+// previous one), and the cells of one step touch no cell another cell of the
+// step reads, so one barrier per step orders everything.
+//
+// Sweeps run in groups of up to D, member s starting LAG = 4 steps after
+// member s - 1 in the same q: member s reads diagonals k - 3 .. k + 1 around
+// its step k while member s - 1 writes k + 2 and k + 4 and member s + 1 writes
+// k - 4 and k - 6, and every value a member reads from the previous sweep is
+// final (its inner front passed it at least one step before), so the group
+// gives exactly the values of its sweeps run one after the other.  The
+// stopping test of each member is known when the group ends: if member s
+// passes it first, q goes back to the snapshot taken at the group's start (in
+// global memory) and a group of s + 1 members runs again, which ends on member
+// s's result.  (Saving each overwritten value instead puts a global store in
+// every step, and on this GPU a wait for a load also waits for the stores
+// issued before it.)  The fold runs per row in the row's own order, then over
+// the threads in order: boundary rows, then inner rows.
 //
 // q lives per row over the row's span [lo, hi] of non-outside cells plus a
 // zero guard cell at each end (row r at spans[3r]: column j at
@@ -387,14 +399,16 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // bounds check).  LDS holds q and the flags (QLDS) or they stay in global
 // memory.  At most max_sweeps sweeps per launch; status = {sweeps so far,
 // converged}; a launch with resume = 1 continues from qspan.
-template <class T, bool QLDS>
-__global__ void __launch_bounds__(1024) k2s_poisson(
+constexpr int kPoisLag = 4;
+
+template <class T, bool QLDS, int D, int MAXT>
+__global__ void __launch_bounds__(MAXT) k2s_poisson(
     int nx, int ny, int S, int rp, const int32_t* __restrict__ spans, const uint8_t* __restrict__ fspan,
-    const T* __restrict__ dv, T* __restrict__ qspan, double dx2, double dy2, double rcp, double tol,
-    int max_sweeps, int resume, int32_t* __restrict__ status) {
+    const T* __restrict__ dv, T* __restrict__ qspan, T* __restrict__ save, double dx2, double dy2,
+    double rcp, double tol, int max_sweeps, int resume, int32_t* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ErrFold sh[16];
-    __shared__ int s_go;
+    __shared__ ErrFold sh[D][16];
+    __shared__ int s_first;
     T* qs = QLDS ? reinterpret_cast<T*>(smem) : qspan;
     const uint8_t* fl = QLDS ? smem + (size_t)S * sizeof(T) : fspan;
     if (QLDS) {
@@ -422,88 +436,133 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
         base[d] = a[d] + 1 - lo[d];
         b[d] = a[d] + (hi[d] - lo[d] + 1) + 1;
     }
-    auto clampi = [](int x, int a, int b) { return min(max(x, a), b); };
+    auto clampi = [](int x, int lo_, int hi_) { return min(max(x, lo_), hi_); };
     auto inside = [&](int d, int j) -> bool { return j >= lo[d] && j <= hi[d]; };
     auto mask = [](T v, bool in) -> double {
         const long long m = in ? -1ll : 0ll;
         return __longlong_as_double(__double_as_longlong((double)v) & m);
     };
     // The step's barrier: with q in LDS nothing global is shared between
-    // threads, so it waits for the LDS traffic only and the div loads of the
-    // next group stay in flight across it.
+    // threads within a group, so it waits for the LDS traffic only and the div
+    // loads and saves stay in flight across it.
     auto step_barrier = [&]() {
         if (QLDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         else __syncthreads();
     };
-    // div of the inner set's cells, loaded a group of GS steps ahead
-    constexpr int GS = 4;
-    double dcur[GS], dnext[GS];
-    auto load_group = [&](int kg, double (&d)[GS]) {
-#pragma unroll
-        for (int s = 0; s < GS; s++) {
-            const int j = kg + s - 2 - r;
-            d[s] = (inner && own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
-        }
+    auto dv_at = [&](int j) -> double {
+        return (inner && own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
     };
+    const int K = nx + ny + 1;
     int sweeps = resume ? status[0] : 0;
-    const int steps = nx + ny + 1;
-    for (int n = 0; n < max_sweeps; n++) {
-        ErrFold fo{0, 0};
-        load_group(0, dcur);
-        for (int kg = 0; kg < steps; kg += GS) {
-            load_group(kg + GS, dnext);
+    int left = max_sweeps;
+    bool converged = false;
+    int rerun = 0;  // members of a group run again from the snapshot (uniform)
+    while (left > 0) {
+        const int g = rerun ? rerun : min(D, left);  // members of this group (uniform)
+        if (!rerun && g > 1)
+            for (int q = threadIdx.x; q < S; q += blockDim.x) save[q] = qs[q];
+        ErrFold fo[D];
 #pragma unroll
-            for (int s = 0; s < GS; s++) {
-                if (kg + s >= steps) break;  // uniform
-                const int j = kg + s - r - 2 * inner;
-                // own row: the cell and its two row neighbours (clamped so the
-                // three reads stay in the row's storage), its flags, and the
-                // neighbour rows at column j, issued together; the empty asm
-                // takes them all, so none is sunk into a branch of its own
-                const int c = clampi(base[1] + j, a[1] + 1, max(b[1] - 1, a[1] + 1));
-                const int ca = clampi(base[0] + j, a[0], b[0]);
-                const int cb = clampi(base[2] + j, a[2], b[2]);
-                T lq = qs[c], ll = qs[c - 1], lr = qs[c + 1], la = qs[ca], lb = qs[cb];
-                int lf = fl[c];
-                asm volatile("" : "+v"(lq), "+v"(ll), "+v"(lr), "+v"(la), "+v"(lb), "+v"(lf));
-                const bool in1 = inside(1, j);
-                const int f = lf & (in1 ? 0xff : 0);
-                const double qo = (double)lq, lf_ = (double)ll, rt = (double)lr;
-                const double up = mask(la, inside(0, j)), dn = mask(lb, inside(2, j));
-                if (!inner) {
-                    if (f & PF_BOUND) {
+        for (int m = 0; m < D; m++) fo[m] = ErrFold{0, 0};
+        const int steps = K + (g - 1) * kPoisLag;
+        // div of each member's inner cell, loaded one step ahead (two buffers)
+        double dA[D], dB[D];
+#pragma unroll
+        for (int m = 0; m < D; m++) {
+            dA[m] = dv_at(0 - m * kPoisLag - 2 - r);
+            dB[m] = 0.0;
+        }
+        for (int k2 = 0; k2 < steps; k2 += 2) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int kk = k2 + h;
+                if (kk >= steps) break;  // uniform
+                double(&dcur)[D] = h == 0 ? dA : dB;
+                double(&dnxt)[D] = h == 0 ? dB : dA;
+                T lq[D], ll[D], lr[D], la[D], lb[D];
+                int lf[D], c[D], j[D];
+                bool act[D];
+#pragma unroll
+                for (int m = 0; m < D; m++) {
+                    const int k = kk - m * kPoisLag;
+                    act[m] = m < g && k >= 0 && k < K;  // uniform
+                    j[m] = k - r - 2 * inner;
+                    dnxt[m] = dv_at(k + 1 - 2 - r);
+                    c[m] = clampi(base[1] + j[m], a[1] + 1, max(b[1] - 1, a[1] + 1));
+                    const int ca = clampi(base[0] + j[m], a[0], b[0]);
+                    const int cb = clampi(base[2] + j[m], a[2], b[2]);
+                    lq[m] = qs[c[m]];
+                    ll[m] = qs[c[m] - 1];
+                    lr[m] = qs[c[m] + 1];
+                    la[m] = qs[ca];
+                    lb[m] = qs[cb];
+                    lf[m] = fl[c[m]];
+                }
+                // the empty asm takes every read, so none is sunk into a branch
+#pragma unroll
+                for (int m = 0; m < D; m++)
+                    asm volatile("" : "+v"(lq[m]), "+v"(ll[m]), "+v"(lr[m]), "+v"(la[m]), "+v"(lb[m]),
+                                 "+v"(lf[m]));
+#pragma unroll
+                for (int m = 0; m < D; m++) {
+                    if (!act[m]) continue;
+                    const int f = lf[m] & (inside(1, j[m]) ? 0xff : 0);
+                    const double qo = (double)lq[m], lf_ = (double)ll[m], rt = (double)lr[m];
+                    const double up = mask(la[m], inside(0, j[m])), dn = mask(lb[m], inside(2, j[m]));
+                    double qn;
+                    bool hit;
+                    if (!inner) {
+                        hit = (f & PF_BOUND) != 0;
                         const double i0 = (f & PF_UP) ? up : dn, i1 = (f & PF_DOWN) ? dn : up;
                         const double j0 = (f & PF_LEFT) ? lf_ : rt, j1 = (f & PF_RIGHT) ? rt : lf_;
                         // div is zero on boundary cells
-                        const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
-                        qs[c] = (T)qn;
-                        fold_cell(fo, qn, qo, tol);
+                        qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
+                    } else {
+                        hit = (f & PF_IN) != 0;
+                        qn = rcp * ((up + dn) * dy2 + (lf_ + rt) * dx2 - dcur[m] * dx2 * dy2);
                     }
-                } else {
-                    if (f & PF_IN) {
-                        const double qn = rcp * ((up + dn) * dy2 + (lf_ + rt) * dx2 - dcur[s] * dx2 * dy2);
-                        qs[c] = (T)qn;
-                        fold_cell(fo, qn, qo, tol);
+                    if (hit) {
+                        qs[c[m]] = (T)qn;
+                        fold_cell(fo[m], qn, qo, tol);
                     }
                 }
                 step_barrier();
             }
-#pragma unroll
-            for (int s = 0; s < GS; s++) dcur[s] = dnext[s];
         }
-        const ErrFold all = fold_block(fo, sh);
-        sweeps++;
+        // the members' stopping tests, in order
+        ErrFold all[D];
+#pragma unroll
+        for (int m = 0; m < D; m++) all[m] = fold_block(fo[m], sh[m]);
         if (threadIdx.x == 0) {
-            const bool done = !(all.kind != 0 && all.ge);
-            s_go = done ? 0 : 1;
-            status[1] = done ? 1 : 0;
+            int first = -1;
+#pragma unroll
+            for (int m = 0; m < D; m++)
+                if (first < 0 && m < g && !(all[m].kind != 0 && all[m].ge)) first = m;
+            s_first = first;
         }
         __syncthreads();
-        if (!s_go) break;
+        const int first = s_first;
+        if (rerun || first + 1 == g) {  // a rerun ends on its last member
+            sweeps += g;
+            converged = true;
+            break;
+        }
+        if (first >= 0) {
+            for (int q = threadIdx.x; q < S; q += blockDim.x) qs[q] = save[q];
+            __syncthreads();
+            rerun = first + 1;
+            continue;
+        }
+        sweeps += g;
+        left -= g;
     }
+    __syncthreads();
     if (QLDS)
-        for (int c = threadIdx.x; c < S; c += blockDim.x) qspan[c] = qs[c];
-    if (threadIdx.x == 0) status[0] = sweeps;
+        for (int q = threadIdx.x; q < S; q += blockDim.x) qspan[q] = qs[q];
+    if (threadIdx.x == 0) {
+        status[0] = sweeps;
+        status[1] = converged ? 1 : 0;
+    }
 }
 
 // q over the whole grid (zero outside the spans) for the gradient.
@@ -551,6 +610,7 @@ class Solver2D final : public ISolver2 {
     // cell flags of the relaxation, its status words
     L3<T> nw{};
     T *d_nw = nullptr, *d_dv = nullptr, *d_qf = nullptr, *d_qspan = nullptr;
+    T* d_save = nullptr;  // q at the start of a relaxation group
     int32_t* d_spans = nullptr;
     uint8_t* d_fspan = nullptr;
     int32_t* d_pstat = nullptr;
@@ -643,8 +703,10 @@ class Solver2D final : public ISolver2 {
         CMCX_HIP(hipStreamSynchronize(st));
         hipFree(d_fspan);
         hipFree(d_qspan);
+        hipFree(d_save);
         d_fspan = nullptr;
         d_qspan = nullptr;
+        d_save = nullptr;
         S = (int)fl.size();
         if (!d_spans) {
             CMCX_HIP(hipMalloc((void**)&d_spans, sizeof(int32_t) * sp.size()));
@@ -659,6 +721,7 @@ class Solver2D final : public ISolver2 {
         }
         CMCX_HIP(hipMalloc((void**)&d_fspan, std::max(S, 1)));
         CMCX_HIP(hipMalloc((void**)&d_qspan, sizeof(T) * std::max(S, 1)));
+        CMCX_HIP(hipMalloc((void**)&d_save, sizeof(T) * std::max(S, 1)));
         CMCX_HIP(hipMemcpyAsync(d_spans, sp.data(), sizeof(int32_t) * sp.size(), hipMemcpyHostToDevice, st));
         if (S) CMCX_HIP(hipMemcpyAsync(d_fspan, fl.data(), S, hipMemcpyHostToDevice, st));
         CMCX_HIP(hipStreamSynchronize(st));
@@ -712,6 +775,7 @@ class Solver2D final : public ISolver2 {
         hipFree(d_dv);
         hipFree(d_qf);
         hipFree(d_qspan);
+        hipFree(d_save);
         hipFree(d_spans);
         hipFree(d_fspan);
         hipFree(d_pstat);
@@ -828,11 +892,22 @@ class Solver2D final : public ISolver2 {
         return (int)sweeps.size();
     }
 
-    template <bool QLDS>
+    // Sweeps per group by work-group size: the group's reads ride in
+    // registers (six per member and cell), so the narrower work-groups (more
+    // registers per lane) take more members.
+    template <bool QLDS, int D, int MAXT>
     void launch_poisson(int rp, size_t lds, double dx2, double dy2, double rcp, int chunk, int resume) {
-        hipLaunchKernelGGL((k2s_poisson<T, QLDS>), dim3(1), dim3(2 * rp), lds, st, nx, ny, S, rp,
-                           d_spans, d_fspan, d_dv, d_qspan, dx2, dy2, rcp, ptol, chunk, resume, d_pstat);
+        hipLaunchKernelGGL((k2s_poisson<T, QLDS, D, MAXT>), dim3(1), dim3(2 * rp), lds, st, nx, ny, S, rp,
+                           d_spans, d_fspan, d_dv, d_qspan, d_save, dx2, dy2, rcp, ptol, chunk, resume,
+                           d_pstat);
         CMCX_LAUNCH_CHECK();
+    }
+
+    template <bool QLDS>
+    void launch_poisson_for(int rp, size_t lds, double dx2, double dy2, double rcp, int chunk, int resume) {
+        if (2 * rp <= 256) launch_poisson<QLDS, 8, 256>(rp, lds, dx2, dy2, rcp, chunk, resume);
+        else if (2 * rp <= 512) launch_poisson<QLDS, 4, 512>(rp, lds, dx2, dy2, rcp, chunk, resume);
+        else launch_poisson<QLDS, 1, 1024>(rp, lds, dx2, dy2, rcp, chunk, resume);
     }
 
     // StableSolver2D::Project (StableSolver2D.cpp:83-165): w -> next on NODE_IN.
@@ -844,13 +919,13 @@ class Solver2D final : public ISolver2 {
         const double dx2 = prm.dx * prm.dx, dy2 = prm.dy * prm.dy, rcp = 0.5 / (dx2 + dy2);
         const int rp = std::max(64, (nx + 63) / 64 * 64);  // threads per set
         const size_t lds = (size_t)S * sizeof(T) + (size_t)S;
-        const bool in_lds = lds + 1024 <= 160 * 1024;
+        const bool in_lds = lds + 2048 <= 160 * 1024;
         int64_t total = 0;
         bool done = false;
         for (int resume = 0; !done && total < pcap; resume = 1) {
             const int chunk = (int)std::min<int64_t>(kPoisChunk, pcap - total);
-            if (in_lds) launch_poisson<true>(rp, lds, dx2, dy2, rcp, chunk, resume);
-            else launch_poisson<false>(rp, 0, dx2, dy2, rcp, chunk, resume);
+            if (in_lds) launch_poisson_for<true>(rp, lds, dx2, dy2, rcp, chunk, resume);
+            else launch_poisson_for<false>(rp, 0, dx2, dy2, rcp, chunk, resume);
             int32_t h[2];
             CMCX_HIP(hipMemcpyAsync(h, d_pstat, sizeof(h), hipMemcpyDeviceToHost, st));
             CMCX_HIP(hipStreamSynchronize(st));
