@@ -267,19 +267,23 @@ __global__ void k2s_solve_uv(int nx, int ny, const uint8_t* __restrict__ type, L
 }
 
 // Project's divergence field (StableSolver2D.cpp:85-97): w.Ux + w.Vy on NODE_IN
-// cells, zero elsewhere.
+// cells, zero elsewhere, stored diagonal by diagonal (cell (i, j) at
+// doff[i + j] + i - max(0, i + j - ny + 1)): the relaxation reads the cells of
+// one anti-diagonal together, which then lie side by side.
 template <class T>
 __global__ void k2s_div(int nx, int ny, const uint8_t* __restrict__ type, L3<T> w, T dx, T dy,
-                        T* __restrict__ dv) {
+                        const int32_t* __restrict__ doff, T* __restrict__ dvd) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (int64_t)nx * ny) return;
+    const int i = (int)(q / ny), j = (int)(q % ny), d = i + j;
+    const int64_t at = doff[d] + i - max(0, d - ny + 1);
     if (type[q] != NODE_IN) {
-        dv[q] = 0;
+        dvd[at] = 0;
         return;
     }
     const T* U = w.f[0];
     const T* V = w.f[1];
-    dv[q] = (U[q + ny] - U[q - ny]) / (2 * dx) + (V[q + 1] - V[q - 1]) / (2 * dy);
+    dvd[at] = (U[q + ny] - U[q - ny]) / (2 * dx) + (V[q + 1] - V[q - 1]) / (2 * dy);
 }
 
 // Project's last loop (StableSolver2D.cpp:153-164): w minus the gradient of q
@@ -367,16 +371,14 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // from its neighbours' current ones (those earlier in the order already
 // relaxed in this sweep), until the stopping test passes.
 //
-// One work-group of two thread sets of rp threads (rp >= nx, a multiple of
-// 64, grids of up to 512 rows): thread r of the first set relaxes row r's
-// boundary cells, thread r of the second its inner cells.  A sweep's step k
-// relaxes the boundary cell (r, k - r) and the inner cell (r, k - 2 - r).
-// Every value a cell reads is then the one the sequential order gives it
-// (boundary cells read inner neighbours from the previous sweep and boundary
-// neighbours from this one; inner cells read the boundary cells of this sweep,
-// inner neighbours before them from this sweep and after them from the
-// previous one), and the cells of one step touch no cell another cell of the
-// step reads, so one barrier per step orders everything.
+// One work-group.  A sweep's step k relaxes the boundary cell (r, k - r) and
+// the inner cell (r, k - 2 - r) of every row r: every value a cell reads is
+// then the one the sequential order gives it (boundary cells read inner
+// neighbours from the previous sweep and boundary neighbours from this one;
+// inner cells read the boundary cells of this sweep, inner neighbours before
+// them from this sweep and after them from the previous one), and the cells of
+// one step touch no cell another cell of the step reads, so one barrier per
+// step orders everything.
 //
 // Sweeps run in groups of up to D, member s starting LAG = 4 steps after
 // member s - 1 in the same q: member s reads diagonals k - 3 .. k + 1 around
@@ -384,13 +386,16 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // k - 4 and k - 6, and every value a member reads from the previous sweep is
 // final (its inner front passed it at least one step before), so the group
 // gives exactly the values of its sweeps run one after the other.  The
-// stopping test of each member is known when the group ends: if member s
-// passes it first, q goes back to the snapshot taken at the group's start (in
-// global memory) and a group of s + 1 members runs again, which ends on member
-// s's result.  (Saving each overwritten value instead puts a global store in
-// every step, and on this GPU a wait for a load also waits for the stores
-// issued before it.)  The fold runs per row in the row's own order, then over
-// the threads in order: boundary rows, then inner rows.
+// work-group holds D slices of rp threads (rp >= nx, a multiple of 64):
+// thread r of a slice relaxes row r's boundary and inner cells for its member
+// (a wave skips the boundary part when none of its rows has a boundary cell
+// at the step); the members' waves share each SIMD and hide one another's
+// latencies.  The stopping test of each
+// member is known when the group ends: if member s passes it first, q goes
+// back to the snapshot taken at the group's start (in global memory) and a
+// group of s + 1 members runs again, which ends on member s's result.  The
+// fold runs per row in the row's own order, then over a slice's threads in
+// order: boundary rows, then inner rows.
 //
 // q lives per row over the row's span [lo, hi] of non-outside cells plus a
 // zero guard cell at each end (row r at spans[3r]: column j at
@@ -401,13 +406,14 @@ enum { PF_IN = 1, PF_BOUND = 2, PF_UP = 4, PF_DOWN = 8, PF_LEFT = 16, PF_RIGHT =
 // converged}; a launch with resume = 1 continues from qspan.
 constexpr int kPoisLag = 4;
 
-template <class T, bool QLDS, int D, int MAXT>
-__global__ void __launch_bounds__(MAXT) k2s_poisson(
-    int nx, int ny, int S, int rp, const int32_t* __restrict__ spans, const uint8_t* __restrict__ fspan,
-    const T* __restrict__ dv, T* __restrict__ qspan, T* __restrict__ save, double dx2, double dy2,
-    double rcp, double tol, int max_sweeps, int resume, int32_t* __restrict__ status) {
+template <class T, bool QLDS>
+__global__ void __launch_bounds__(1024) k2s_poisson(
+    int nx, int ny, int S, int rp, int D, const int32_t* __restrict__ spans,
+    const uint8_t* __restrict__ fspan, const int32_t* __restrict__ doff, const T* __restrict__ dvd,
+    T* __restrict__ qspan, T* __restrict__ save, double dx2, double dy2, double rcp, double tol,
+    int max_sweeps, int resume, int32_t* __restrict__ status) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ErrFold sh[D][16];
+    __shared__ ErrFold sh[32];
     __shared__ int s_first;
     T* qs = QLDS ? reinterpret_cast<T*>(smem) : qspan;
     const uint8_t* fl = QLDS ? smem + (size_t)S * sizeof(T) : fspan;
@@ -420,8 +426,8 @@ __global__ void __launch_bounds__(MAXT) k2s_poisson(
         for (int c = threadIdx.x; c < S; c += blockDim.x) qs[c] = (T)0;
     }
     __syncthreads();
-    const int inner = (int)threadIdx.x >= rp ? 1 : 0;  // uniform per wave
-    const int r = (int)threadIdx.x - inner * rp;
+    const int mem = (int)threadIdx.x / rp;  // member slice, uniform per wave
+    const int r = (int)threadIdx.x - mem * rp;
     const bool own = r < nx;
     // padded spans of rows r - 1, r, r + 1: storage [a, b] holds columns
     // lo - 1 .. hi + 1 (column j at base + j); a cell is inside for lo <= j <= hi
@@ -444,15 +450,21 @@ __global__ void __launch_bounds__(MAXT) k2s_poisson(
     };
     // The step's barrier: with q in LDS nothing global is shared between
     // threads within a group, so it waits for the LDS traffic only and the div
-    // loads and saves stay in flight across it.
+    // loads stay in flight across it.
     auto step_barrier = [&]() {
         if (QLDS) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
         else __syncthreads();
     };
+    // div of (r, j), diagonal-major: the lanes of a wave read one diagonal's
+    // consecutive cells
     auto dv_at = [&](int j) -> double {
-        return (inner && own && j >= 0 && j < ny) ? (double)dv[(int64_t)r * ny + j] : 0.0;
+        const bool ok = own && j >= 0 && j < ny;
+        const int d = ok ? r + j : 0;  // every index stays inside its array
+        const int at = ok ? doff[d] + r - max(0, d - ny + 1) : 0;
+        return ok ? (double)dvd[at] : 0.0;
     };
     const int K = nx + ny + 1;
+    const int wpm = rp / 64;  // waves per member slice
     int sweeps = resume ? status[0] : 0;
     int left = max_sweeps;
     bool converged = false;
@@ -461,83 +473,86 @@ __global__ void __launch_bounds__(MAXT) k2s_poisson(
         const int g = rerun ? rerun : min(D, left);  // members of this group (uniform)
         if (!rerun && g > 1)
             for (int q = threadIdx.x; q < S; q += blockDim.x) save[q] = qs[q];
-        ErrFold fo[D];
-#pragma unroll
-        for (int m = 0; m < D; m++) fo[m] = ErrFold{0, 0};
+        ErrFold fb{0, 0}, fi{0, 0};
         const int steps = K + (g - 1) * kPoisLag;
-        // div of each member's inner cell, loaded one step ahead (two buffers)
-        double dA[D], dB[D];
-#pragma unroll
-        for (int m = 0; m < D; m++) {
-            dA[m] = dv_at(0 - m * kPoisLag - 2 - r);
-            dB[m] = 0.0;
-        }
+        const int k0 = mem * kPoisLag;  // this member's first step
+        double dA = dv_at(0 - k0 - 2 - r), dB = 0.0;
         for (int k2 = 0; k2 < steps; k2 += 2) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int kk = k2 + h;
                 if (kk >= steps) break;  // uniform
-                double(&dcur)[D] = h == 0 ? dA : dB;
-                double(&dnxt)[D] = h == 0 ? dB : dA;
-                T lq[D], ll[D], lr[D], la[D], lb[D];
-                int lf[D], c[D], j[D];
-                bool act[D];
-#pragma unroll
-                for (int m = 0; m < D; m++) {
-                    const int k = kk - m * kPoisLag;
-                    act[m] = m < g && k >= 0 && k < K;  // uniform
-                    j[m] = k - r - 2 * inner;
-                    dnxt[m] = dv_at(k + 1 - 2 - r);
-                    c[m] = clampi(base[1] + j[m], a[1] + 1, max(b[1] - 1, a[1] + 1));
-                    const int ca = clampi(base[0] + j[m], a[0], b[0]);
-                    const int cb = clampi(base[2] + j[m], a[2], b[2]);
-                    lq[m] = qs[c[m]];
-                    ll[m] = qs[c[m] - 1];
-                    lr[m] = qs[c[m] + 1];
-                    la[m] = qs[ca];
-                    lb[m] = qs[cb];
-                    lf[m] = fl[c[m]];
-                }
+                double& dcur = h == 0 ? dA : dB;
+                double& dnxt = h == 0 ? dB : dA;
+                const int k = kk - k0;
+                const bool act = mem < g && k >= 0 && k < K;  // uniform per wave
+                const int jb = k - r, ji = jb - 2;
+                dnxt = dv_at(k + 1 - 2 - r);
+                const int own_lo = a[1] + 1, own_hi = max(b[1] - 1, a[1] + 1);
+                const int cb = clampi(base[1] + jb, own_lo, own_hi);
+                const int ci = clampi(base[1] + ji, own_lo, own_hi);
+                const int ub = clampi(base[0] + jb, a[0], b[0]), db = clampi(base[2] + jb, a[2], b[2]);
+                const int ui = clampi(base[0] + ji, a[0], b[0]), di = clampi(base[2] + ji, a[2], b[2]);
+                T bq = qs[cb], bl = qs[cb - 1], br = qs[cb + 1], bu = qs[ub], bd = qs[db];
+                T iq = qs[ci], il = qs[ci - 1], ir = qs[ci + 1], iu = qs[ui], id = qs[di];
+                int bf = fl[cb], jf = fl[ci];
                 // the empty asm takes every read, so none is sunk into a branch
-#pragma unroll
-                for (int m = 0; m < D; m++)
-                    asm volatile("" : "+v"(lq[m]), "+v"(ll[m]), "+v"(lr[m]), "+v"(la[m]), "+v"(lb[m]),
-                                 "+v"(lf[m]));
-#pragma unroll
-                for (int m = 0; m < D; m++) {
-                    if (!act[m]) continue;
-                    const int f = lf[m] & (inside(1, j[m]) ? 0xff : 0);
-                    const double qo = (double)lq[m], lf_ = (double)ll[m], rt = (double)lr[m];
-                    const double up = mask(la[m], inside(0, j[m])), dn = mask(lb[m], inside(2, j[m]));
-                    double qn;
-                    bool hit;
-                    if (!inner) {
-                        hit = (f & PF_BOUND) != 0;
+                asm volatile("" : "+v"(bq), "+v"(bl), "+v"(br), "+v"(bu), "+v"(bd), "+v"(bf));
+                asm volatile("" : "+v"(iq), "+v"(il), "+v"(ir), "+v"(iu), "+v"(id), "+v"(jf));
+                if (act) {
+                    const int f = bf & (inside(1, jb) ? 0xff : 0);
+                    if (f & PF_BOUND) {
+                        const double up = mask(bu, inside(0, jb)), dn = mask(bd, inside(2, jb));
+                        const double lf_ = (double)bl, rt = (double)br;
                         const double i0 = (f & PF_UP) ? up : dn, i1 = (f & PF_DOWN) ? dn : up;
                         const double j0 = (f & PF_LEFT) ? lf_ : rt, j1 = (f & PF_RIGHT) ? rt : lf_;
                         // div is zero on boundary cells
-                        qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
-                    } else {
-                        hit = (f & PF_IN) != 0;
-                        qn = rcp * ((up + dn) * dy2 + (lf_ + rt) * dx2 - dcur[m] * dx2 * dy2);
+                        const double qn = rcp * ((i0 + i1) * dy2 + (j0 + j1) * dx2 - 0.0 * dx2 * dy2);
+                        qs[cb] = (T)qn;
+                        fold_cell(fb, qn, (double)bq, tol);
                     }
-                    if (hit) {
-                        qs[c[m]] = (T)qn;
-                        fold_cell(fo[m], qn, qo, tol);
+                    const int g2 = jf & (inside(1, ji) ? 0xff : 0);
+                    if (g2 & PF_IN) {
+                        const double up = mask(iu, inside(0, ji)), dn = mask(id, inside(2, ji));
+                        const double qn =
+                            rcp * ((up + dn) * dy2 + ((double)il + (double)ir) * dx2 - dcur * dx2 * dy2);
+                        qs[ci] = (T)qn;
+                        fold_cell(fi, qn, (double)iq, tol);
                     }
                 }
                 step_barrier();
             }
         }
-        // the members' stopping tests, in order
-        ErrFold all[D];
-#pragma unroll
-        for (int m = 0; m < D; m++) all[m] = fold_block(fo[m], sh[m]);
+        // each member's stopping test: an ordered fold over its slice's
+        // waves, boundary rows then inner rows
+        {
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            ErrFold vb = fb, vi = fi;
+            for (int off = 1; off < 64; off <<= 1) {
+                ErrFold ob, oi;
+                ob.kind = __shfl_down(vb.kind, off);
+                ob.ge = __shfl_down(vb.ge, off);
+                oi.kind = __shfl_down(vi.kind, off);
+                oi.ge = __shfl_down(vi.ge, off);
+                if (lane + off < 64) {
+                    vb = fold_cat(vb, ob);
+                    vi = fold_cat(vi, oi);
+                }
+            }
+            if (lane == 0) {
+                sh[wave] = vb;
+                sh[16 + wave] = vi;
+            }
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
             int first = -1;
-#pragma unroll
-            for (int m = 0; m < D; m++)
-                if (first < 0 && m < g && !(all[m].kind != 0 && all[m].ge)) first = m;
+            for (int m = 0; m < g && first < 0; m++) {
+                ErrFold all{0, 0};
+                for (int w = 0; w < wpm; w++) all = fold_cat(all, sh[m * wpm + w]);
+                for (int w = 0; w < wpm; w++) all = fold_cat(all, sh[16 + m * wpm + w]);
+                if (!(all.kind != 0 && all.ge)) first = m;
+            }
             s_first = first;
         }
         __syncthreads();
@@ -612,6 +627,7 @@ class Solver2D final : public ISolver2 {
     T *d_nw = nullptr, *d_dv = nullptr, *d_qf = nullptr, *d_qspan = nullptr;
     T* d_save = nullptr;  // q at the start of a relaxation group
     int32_t* d_spans = nullptr;
+    int32_t* d_doff = nullptr;  // first cell of each anti-diagonal, diagonal-major
     uint8_t* d_fspan = nullptr;
     int32_t* d_pstat = nullptr;
     int S = 0;
@@ -709,6 +725,13 @@ class Solver2D final : public ISolver2 {
         d_save = nullptr;
         S = (int)fl.size();
         if (!d_spans) {
+            std::vector<int32_t> doff(nx + ny);
+            for (int d = 0, o = 0; d < nx + ny; d++) {
+                doff[d] = o;
+                if (d < nx + ny - 1) o += std::min(d, nx - 1) - std::max(0, d - ny + 1) + 1;
+            }
+            CMCX_HIP(hipMalloc((void**)&d_doff, sizeof(int32_t) * doff.size()));
+            CMCX_HIP(hipMemcpy(d_doff, doff.data(), sizeof(int32_t) * doff.size(), hipMemcpyHostToDevice));
             CMCX_HIP(hipMalloc((void**)&d_spans, sizeof(int32_t) * sp.size()));
             CMCX_HIP(hipMalloc((void**)&d_nw, sizeof(T) * N * 2));
             CMCX_HIP(hipMalloc((void**)&d_dv, sizeof(T) * N));
@@ -776,6 +799,7 @@ class Solver2D final : public ISolver2 {
         hipFree(d_qf);
         hipFree(d_qspan);
         hipFree(d_save);
+        hipFree(d_doff);
         hipFree(d_spans);
         hipFree(d_fspan);
         hipFree(d_pstat);
@@ -892,29 +916,22 @@ class Solver2D final : public ISolver2 {
         return (int)sweeps.size();
     }
 
-    // Sweeps per group by work-group size: the group's reads ride in
-    // registers (six per member and cell), so the narrower work-groups (more
-    // registers per lane) take more members.
-    template <bool QLDS, int D, int MAXT>
-    void launch_poisson(int rp, size_t lds, double dx2, double dy2, double rcp, int chunk, int resume) {
-        hipLaunchKernelGGL((k2s_poisson<T, QLDS, D, MAXT>), dim3(1), dim3(2 * rp), lds, st, nx, ny, S, rp,
-                           d_spans, d_fspan, d_dv, d_qspan, d_save, dx2, dy2, rcp, ptol, chunk, resume,
-                           d_pstat);
-        CMCX_LAUNCH_CHECK();
-    }
-
+    // Sweeps in flight: as many member slices of rp threads as a 1024-thread
+    // work-group holds, at most 8.
     template <bool QLDS>
     void launch_poisson_for(int rp, size_t lds, double dx2, double dy2, double rcp, int chunk, int resume) {
-        if (2 * rp <= 256) launch_poisson<QLDS, 8, 256>(rp, lds, dx2, dy2, rcp, chunk, resume);
-        else if (2 * rp <= 512) launch_poisson<QLDS, 4, 512>(rp, lds, dx2, dy2, rcp, chunk, resume);
-        else launch_poisson<QLDS, 1, 1024>(rp, lds, dx2, dy2, rcp, chunk, resume);
+        const int D = std::min(8, 1024 / rp);
+        hipLaunchKernelGGL((k2s_poisson<T, QLDS>), dim3(1), dim3(rp * D), lds, st, nx, ny, S, rp, D,
+                           d_spans, d_fspan, d_doff, d_dv, d_qspan, d_save, dx2, dy2, rcp, ptol, chunk,
+                           resume, d_pstat);
+        CMCX_LAUNCH_CHECK();
     }
 
     // StableSolver2D::Project (StableSolver2D.cpp:83-165): w -> next on NODE_IN.
     // Returns false when the relaxation reached the sweep cap.
     bool project() {
         hipLaunchKernelGGL((k2s_div<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, d_type, nw,
-                           (T)prm.dx, (T)prm.dy, d_dv);
+                           (T)prm.dx, (T)prm.dy, d_doff, d_dv);
         CMCX_LAUNCH_CHECK();
         const double dx2 = prm.dx * prm.dx, dy2 = prm.dy * prm.dy, rcp = 0.5 / (dx2 + dy2);
         const int rp = std::max(64, (nx + 63) / 64 * 64);  // threads per set
