@@ -1187,10 +1187,16 @@ class Solver final : public ISolver {
                 const int k = 3 * it + q, a = axes[q];
                 Layer4<T> tin = lay(sb, k == 0 ? 0 : k), out = lay(sb, 1 + 3 * G + k);
                 const T alpha = q == 2 ? (T)0.25 : (T)0.5, beta = q == 2 ? (T)0.75 : (T)0.5;
-                launch_adj_sweep<T>(g, a, sweep_params(0, a, adj.dt), d_type, d_bcv, d_bct, tin, out,
-                                    XB, TB, CB, LB, GB, alpha, beta, q == 0 ? 1 : 0,
-                                    (q == 0 && it == 0) ? 1 : 0, adj.tiles[a], adj.ntiles[a],
-                                    adj.ltiles[a], adj.pp, st);
+                const SweepParams<T> sp = sweep_params(0, a, adj.dt);
+                // adjoint of the sweep output: the step's seed (XB) for its last
+                // X sweep, zero for the other X sweeps (their output feeds only
+                // the Picard layer), (3/dt) lam of the sweep reversed before
+                const Layer4<T> xs = q == 2 ? (it == G - 1 ? XB : Layer4<T>{}) : LB;
+                const T xscale = q == 2 ? (T)1 : sp.three_dt;
+                launch_adj_sweep<T>(g, a, sp, d_type, d_bcv, d_bct, tin, out, xs, xscale, XB, TB, CB,
+                                    LB, GB, alpha, beta, q == 0 ? 1 : 0,
+                                    (q == 0 && it == G - 1) ? 1 : 0, (q == 0 && it == 0) ? 1 : 0,
+                                    adj.tiles[a], adj.ntiles[a], adj.ltiles[a], adj.pp, st);
             }
     }
 
@@ -1210,7 +1216,7 @@ class Solver final : public ISolver {
             CMCX_HIP(hipMalloc((void**)&adj.work, sizeof(T) * 4 * NT * 6));
             CMCX_HIP(hipMalloc((void**)&adj.pp, sizeof(double) * 4 * adj_point_blocks()));
             for (int a = 0; a < 3; a++)
-                build_adj_tiles(h_type.data(), g, a, adj_lpg<T>(), &adj.tiles[a], &adj.ntiles[a],
+                build_adj_tiles(h_type.data(), g, a, adj_lpg<T>(g), &adj.tiles[a], &adj.ntiles[a],
                                 &adj.ltiles[a]);
         }
         Layer4<T> XB = lay(adj.work, 0), GB = lay(adj.work, 4), FIN = lay(adj.work, 5);

@@ -148,9 +148,9 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
 int shared_free_axes(const uint8_t* type, const uint8_t* bcv, const uint8_t* bct, const Geom& g);
 
 // Adjoint sweeps (kernels_adjoint.hip).  Lines per tile of the transposed line
-// solve: 128 bytes of lines (adj_lpg); every axis at most 256 nodes long.
+// solve (adj_lpg: 128 or 64 bytes of lines); every axis at most 256 nodes long.
 template <class T>
-int adj_lpg();
+int adj_lpg(const Geom& g);
 template <class T>
 bool adjoint_supported(const Geom& g);
 // Tiles (outer * ltiles + line tile) of `axis` holding a NODE_IN node, from host types.
@@ -158,18 +158,21 @@ void build_adj_tiles(const uint8_t* h_type, const Geom& g, int axis, int lpg, in
                      int64_t* ntiles, int* ltiles);
 // Number of per-block parameter partial slots (4 doubles each) of launch_adj_sweep.
 int adj_point_blocks();
-// Reverses one sweep: xb/tb (adjoint of the sweep output / of the Picard layer
-// after it) -> lam, tb (before it), xb (adjoint of the sweep input) or, with
-// zmode, cacc += input adjoint and xb = 0; gb += lam on Dirichlet end rows;
-// pp[4*b..] += parameter partials (v_T, v_vis, t_vis, t_phi).  `last` (the Z
-// sweep of the first Picard iteration) also ends the reversed step: xb = state
-// adjoint, gb += tbar on BOUND/VALVE nodes, tb and cacc cleared.
+// Reverses one sweep: xscale * xs (adjoint of the sweep output; xs.f[0] null
+// for zero; xs may be `lam`, read before it is overwritten) and tb (adjoint of
+// the Picard layer after the sweep) -> lam, tb (before it); the input adjoint
+// is (3/dt) lam on NODE_IN nodes, left in lam for the next reversed sweep, or
+// with zmode (the input is the state) added to cacc (first_z: cacc set);
+// gb += lam on the node-value rows; pp[4*b..] += parameter partials (v_T,
+// v_vis, t_vis, t_phi).  `last` (the Z sweep of the first Picard iteration)
+// also ends the reversed step: xb = state adjoint, gb += tbar on BOUND/VALVE
+// nodes, tb cleared.
 template <class T>
 void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
-                      Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc, Layer4<T> lam, Layer4<T> gb,
-                      T alpha, T beta, int zmode, int last, const int32_t* tiles, int64_t ntiles,
-                      int ltiles, double* pp, hipStream_t st);
+                      Layer4<T> xs, T xscale, Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc,
+                      Layer4<T> lam, Layer4<T> gb, T alpha, T beta, int zmode, int first_z, int last,
+                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp, hipStream_t st);
 // dst <- src, four fields of n elements, one launch.
 template <class T>
 void launch_copy4(Layer4<T> dst, Layer4<T> src, int64_t n, hipStream_t st);

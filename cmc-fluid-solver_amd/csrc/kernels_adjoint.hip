@@ -10,13 +10,19 @@
 //              end rows; lam on Dirichlet end rows is the node-value gradient.
 //              Row p of M^T is (c_{p-1}, b_p, a_{p+1}): the coefficients of the
 //              neighbouring rows are rebuilt from their roles and temp.
+//              xbar is never stored between sweeps: it is the adjoint of the
+//              next sweep's input, (3/dt) lam of the sweep reversed before (read
+//              from lam, which this solve then overwrites row by row in the
+//              same thread), zero after a Z sweep (whose input is the state),
+//              or the seed / the previous reversed step's state adjoint.
 //   adj_point  (pointwise)  with Mbar = -lam x^T on the tridiagonal pattern:
 //              tbar <- alpha tbar (NODE_IN) + d(a, c)/dw : Mbar (advecting
 //              velocity) + the transposed pressure and dissipation stencils of
 //              the neighbouring rows (a gather: each node collects the terms of
 //              the rows that read it, so no two threads write one node);
-//              inbar = (3/dt) lam (the adjoint of the sweep input, or added to
-//              the state adjoint for a Z sweep, whose input is the state);
+//              inbar = (3/dt) lam (the adjoint of the sweep input, taken by
+//              the next solve from lam, or for a Z sweep, whose input is the
+//              state, added to the state adjoint's accumulator);
 //              per-block partial sums of the FluidParams gradients.
 //   end of a reversed step (folded into the last adj_point of the step):
 //              state adjoint = (Z inputs) + tbar on NODE_IN nodes; tbar on
@@ -29,6 +35,7 @@
 #include "line_solve.h"
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 namespace cmcx {
@@ -46,23 +53,22 @@ struct Coef {
 };
 
 // Coefficients of one row of M (both matrices): interior row from the advection
-// term adv = w / 2h, boundary and identity rows from the role and BC kinds.
+// term adv = w / 2h, boundary and identity rows from the role and BC kinds (the
+// rows of bc_row: a free start (0, 2, -1), a free end (-1, 2, 0), else the
+// identity).  Written with selects only: a branch here would pull the loads of
+// w and the BC kinds into it, each waited for on its own.
 template <class T>
 __device__ __forceinline__ Coef<T> row_coef(int role, bool fv, bool ft, T adv,
                                             const SweepParams<T>& p) {
+    const bool in = role == RL_IN, st = role == RL_START, en = role == RL_END;
+    const bool sv = st & fv, ev = en & fv, s_t = st & ft, e_t = en & ft;
     Coef<T> k;
-    if (role == RL_IN) {
-        k.av = -adv - p.vis_v;
-        k.cv = adv - p.vis_v;
-        k.bv = p.three_dt + 2 * p.vis_v;
-        k.at = -adv - p.vis_t;
-        k.ct = adv - p.vis_t;
-        k.bt = p.three_dt + 2 * p.vis_t;
-    } else {
-        bool u;
-        bc_row<T>(role, role != RL_NONE && fv, k.av, k.bv, k.cv, u);
-        bc_row<T>(role, role != RL_NONE && ft, k.at, k.bt, k.ct, u);
-    }
+    k.av = in ? -adv - p.vis_v : (ev ? (T)-1 : (T)0);
+    k.cv = in ? adv - p.vis_v : (sv ? (T)-1 : (T)0);
+    k.bv = in ? p.three_dt + 2 * p.vis_v : ((sv | ev) ? (T)2 : (T)1);
+    k.at = in ? -adv - p.vis_t : (e_t ? (T)-1 : (T)0);
+    k.ct = in ? adv - p.vis_t : (s_t ? (T)-1 : (T)0);
+    k.bt = in ? p.three_dt + 2 * p.vis_t : ((s_t | e_t) ? (T)2 : (T)1);
     return k;
 }
 
@@ -72,8 +78,8 @@ __device__ __forceinline__ Coef<T> row_coef(int role, bool fv, bool ft, T adv,
 template <class T, int AXIS, int M, int LPG, int MAXT>
 __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
-    const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xb, Layer4<T> tb, T beta,
-    Layer4<T> lam, Layer4<T> gb, const int32_t* __restrict__ tiles, int ltiles) {
+    const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xs, T xscale, Layer4<T> tb, T beta,
+    Layer4<T> lam, const int32_t* __restrict__ tiles, int ltiles) {
     extern __shared__ char smem_raw[];
     T* red = reinterpret_cast<T*>(smem_raw);
     constexpr int GPW = 64 / LPG;
@@ -100,28 +106,50 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
     }
     const int r0 = chunk * M;
 
-    // node types of rows r0-2 .. r0+M+1, roles of rows r0-1 .. r0+M
+    // Every load of the thread goes out in one batch: node types of rows
+    // r0-2 .. r0+M+1, the advecting velocity and BC kinds of rows r0-1 .. r0+M,
+    // xbar and tbar of rows r0 .. r0+M-1, from addresses clamped to node 0
+    // outside the line and selected afterwards.  A value used under a condition
+    // only would be loaded there, behind its own wait: the velocity and the BC
+    // kinds are therefore used once, all together, right after the loads.
     uint8_t ty[M + 4];
 #pragma unroll
     for (int j = 0; j < M + 4; j++) {
         const int r = r0 + j - 2;
-        ty[j] = (ok && r >= 0 && r < n) ? type[base + (int64_t)r * s] : kOutside;
+        const bool inr = ok && r >= 0 && r < n;
+        const uint8_t t = type[inr ? base + (int64_t)r * s : 0];
+        ty[j] = inr ? t : kOutside;
     }
+    const T* wv = tin.f[AXIS];
+    T wr[M + 2];
+    int fr[M + 2];  // bit 0: velocity BC free, bit 1: temperature BC free
+#pragma unroll
+    for (int j = 0; j < M + 2; j++) {
+        const int r = r0 + j - 1;
+        const int64_t c = (ok && r >= 0 && r < n) ? base + (int64_t)r * s : 0;
+        wr[j] = wv[c];
+        fr[j] = (bcv[c] == BC_FREE ? 1 : 0) | (bct[c] == BC_FREE ? 2 : 0);
+    }
+    const bool hasx = xs.f[0] != nullptr;  // xbar = xscale * xs, or zero
+    T xr[4][M], tr[4][M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        const int64_t c = (ok && r0 + j < n) ? base + (int64_t)(r0 + j) * s : 0;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            xr[v][j] = (hasx ? xs.f[v] : tb.f[v])[c];
+            tr[v][j] = tb.f[v][c];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M + 2; j++) asm volatile("" ::"v"(wr[j]), "v"(fr[j]));
     int role[M + 2];
 #pragma unroll
     for (int j = 0; j < M + 2; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
-    const T* wv = tin.f[AXIS];
     auto coef = [&](int j) {  // row r0 + j, j in [-1, M]
         const int rl = role[j + 1];
-        const int64_t c = base + (int64_t)(r0 + j) * s;
-        T adv = 0;
-        bool fv = false, ft = false;
-        if (rl == RL_IN) adv = wv[c] * p.inv_2h;
-        else if (rl != RL_NONE) {
-            fv = bcv[c] == BC_FREE;
-            ft = bct[c] == BC_FREE;
-        }
-        return row_coef<T>(rl, fv, ft, adv, p);
+        return row_coef<T>(rl, fr[j + 1] & 1, fr[j + 1] & 2,
+                           rl == RL_IN ? wr[j + 1] * p.inv_2h : (T)0, p);
     };
 
     Chunk<T, M> ch;
@@ -133,13 +161,10 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
         Row<T> r;
         r.av = cm.cv; r.bv = c0.bv; r.cv = cp.av;
         r.at = cm.ct; r.bt = c0.bt; r.ct = cp.at;
-        const int64_t c = base + (int64_t)(r0 + j) * s;
-        if (role[j + 1] == RL_IN) {
 #pragma unroll
-            for (int v = 0; v < 4; v++) r.d[v] = xb.f[v][c] + beta * tb.f[v][c];
-        } else {
-#pragma unroll
-            for (int v = 0; v < 4; v++) r.d[v] = 0;
+        for (int v = 0; v < 4; v++) {
+            const T xv = hasx ? xscale * xr[v][j] : (T)0;
+            r.d[v] = role[j + 1] == RL_IN ? xv + beta * tr[v][j] : (T)0;
         }
         if (j == 0) rfirst = r;
         else if (j == M - 1) rlast = r;
@@ -172,11 +197,6 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
         const int rl = role[j + 1];
         if (rl == RL_NONE) continue;
         const int64_t c = base + (int64_t)(r0 + j) * s;
-        bool uv = rl != RL_IN, ut = rl != RL_IN;
-        if (rl == RL_START || rl == RL_END) {
-            uv = bcv[c] != BC_FREE;
-            ut = bct[c] != BC_FREE;
-        }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             T x;
@@ -188,7 +208,6 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
                 x = ch.E[v][j] + F * xf[v] + G * xl[v];
             }
             lam.f[v][c] = x;
-            if ((v < 3 && uv) || (v == 3 && ut)) gb.f[v][c] += x;
         }
     }
 }
@@ -201,8 +220,8 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
 template <class T, int M>
 __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
-    const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xb, Layer4<T> tb, T beta,
-    Layer4<T> lam, Layer4<T> gb, const int32_t* __restrict__ lines, int vec_ok) {
+    const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xs, T xscale, Layer4<T> tb, T beta,
+    Layer4<T> lam, const int32_t* __restrict__ lines, int vec_ok) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -236,12 +255,10 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     Coef<T> k[M];
 #pragma unroll
     for (int j = 0; j < M; j++) {
-        bool fv = false, ft = false;
-        if (role[j] != RL_IN && role[j] != RL_NONE) {
-            fv = bcv[c0 + j] == BC_FREE;
-            ft = bct[c0 + j] == BC_FREE;
-        }
-        k[j] = row_coef<T>(role[j], fv, ft, role[j] == RL_IN ? w[j] * p.inv_2h : (T)0, p);
+        const int64_t c = j < valid ? c0 + j : 0;
+        const uint8_t bv = bcv[c], bt = bct[c];
+        k[j] = row_coef<T>(role[j], bv == BC_FREE, bt == BC_FREE,
+                           role[j] == RL_IN ? w[j] * p.inv_2h : (T)0, p);
     }
     // c of row r0-1 and a of row r0+M (zero outside the line: identity rows)
     T cvm = __shfl(k[M - 1].cv, lane - 1, 64), ctm = __shfl(k[M - 1].ct, lane - 1, 64);
@@ -253,7 +270,14 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
 #pragma unroll
     for (int v = 0; v < 4; v++) {
         T xa[M], ta[M];
-        load_run<T, M>(xb.f[v] + c0, xa, vec, valid);
+        if (xs.f[0] != nullptr) {  // xbar = xscale * xs, or zero
+            load_run<T, M>(xs.f[v] + c0, xa, vec, valid);
+#pragma unroll
+            for (int j = 0; j < M; j++) xa[j] *= xscale;
+        } else {
+#pragma unroll
+            for (int j = 0; j < M; j++) xa[j] = 0;
+        }
         load_run<T, M>(tb.f[v] + c0, ta, vec, valid);
 #pragma unroll
         for (int j = 0; j < M; j++) rhs[v][j] = role[j] == RL_IN ? xa[j] + beta * ta[j] : (T)0;
@@ -299,174 +323,230 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
             }
         }
         store_run<T, M>(lam.f[v] + c0, x, vec, valid);
-#pragma unroll
-        for (int j = 0; j < M; j++) {
-            const int rl = role[j];
-            if (rl == RL_IN || rl == RL_NONE || j >= valid) continue;
-            bool use = true;
-            if (rl == RL_START || rl == RL_END) use = (v < 3 ? bcv[c0 + j] : bct[c0 + j]) != BC_FREE;
-            if (use) gb.f[v][c0 + j] += x[j];
-        }
     }
 }
 
-// Pointwise part of a reversed sweep (see the file comment).  Grid-stride over
-// all nodes with a fixed grid, so that block b always owns partial-sum slot b.
+// Pointwise part of a reversed sweep (see the file comment).  A fixed grid, so
+// that block b always owns partial-sum slot b; each block takes a contiguous
+// run of nodes, and the blocks of one XCD (b mod 8) take neighbouring runs, so
+// the stencil neighbours (x-planes either side) are mostly in that XCD's L2.
+// Every load of a node goes out in one batch: an address is q + an offset of
+// at most two nodes per axis, replaced by q itself when it leaves the grid (so
+// no address waits for a node type), and the terms that the node types rule
+// out are dropped by selects; the stores come last.
+//
+// With a the sweep axis and b, c the other two, a NODE_IN row r reads the
+// dissipation phi(r) = sum_f (D_a F_f)^2 + (D_a F_a)^2 + sum_{o != a} D_a F_o D_o F_a
+// (F = U, V, W; D_o the central difference along o at r), so node q collects,
+// from the NODE_IN rows r = q -+ e_o (sign +-1):
+//   o = a:  F_a += 4 D_a F_a / 2h_a,  F_b += (2 D_a F_b + D_b F_a) / 2h_a  (and
+//           c alike), all times t_phi lam_T(r); T += -v_T lam_a(r) / 2h_a
+//   o != a: F_a += t_phi lam_T(r) D_a F_o(r) / 2h_o.
+// The node-value gradient of the sweep's boundary rows (lam on the rows whose
+// RHS is a node value: Dirichlet ends and shared rows) is added here too.
 template <class T, int AXIS>
 __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p, T inv_h2,
                                                         const uint8_t* __restrict__ type,
+                                                        const uint8_t* __restrict__ bcv,
+                                                        const uint8_t* __restrict__ bct,
                                                         Layer4<T> tin, Layer4<T> out, Layer4<T> lam,
                                                         Layer4<T> tb, Layer4<T> xb, Layer4<T> cacc,
-                                                        Layer4<T> gb, int zmode, int last, T alpha,
-                                                        double* __restrict__ pp) {
+                                                        Layer4<T> gb, int zmode, int first_z,
+                                                        int last, T alpha, double* __restrict__ pp) {
+    constexpr int A = AXIS;
     // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit indices
-    const uint32_t sx = (uint32_t)g.sx, sy = (uint32_t)g.sy;
-    const uint32_t s = AXIS == AX_X ? sx : AXIS == AX_Y ? sy : 1u;
-    const T ix = p.inv_2hx, iy = p.inv_2hy, iz = p.inv_2hz;
-    const T* U = tin.f[0];
-    const T* V = tin.f[1];
-    const T* W = tin.f[2];
+    const int sx = (int)g.sx, sy = (int)g.sy;
+    const int so[3] = {sx, sy, 1};
+    const int dim[3] = {g.nx, g.ny, g.nz};
+    const T io[3] = {p.inv_2hx, p.inv_2hy, p.inv_2hz};
+    const T ia = io[A];
+    const T* F[3] = {tin.f[0], tin.f[1], tin.f[2]};
     const T* Tt = tin.f[3];
     double part[4] = {0, 0, 0, 0};
     const uint32_t N = (uint32_t)g.N;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < N; q += gridDim.x * blockDim.x) {
-        const uint32_t ux = q / sx, rem = q - ux * sx, uy = rem / sy;
-        const int x = (int)ux, y = (int)uy, z = (int)(rem - uy * sy);
-        const uint8_t tq = type[q];
-        const bool inq = tq == NODE_IN;
-        T acc[4], acc_state[4] = {0, 0, 0, 0};
+    const uint32_t per = ((N + gridDim.x - 1) / gridDim.x + 63) & ~63u;
+    const uint32_t q0 = xcd_remap(blockIdx.x, gridDim.x) * per;
+    const uint32_t q1 = min(N, q0 + per);
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const int ux = (int)(q / (uint32_t)sx), rem = (int)q - ux * sx, uy = rem / sy;
+        const int pos[3] = {ux, uy, rem - uy * sy};
+        // q + k1 e_o1 + k2 e_o2 inside the grid, else q
+        auto at = [&](int o1, int k1, int o2 = 0, int k2 = 0) -> uint32_t {
+            int d[3] = {0, 0, 0};
+            d[o1] += k1;
+            d[o2] += k2;
+            const bool in = (unsigned)(pos[0] + d[0]) < (unsigned)dim[0] &&
+                            (unsigned)(pos[1] + d[1]) < (unsigned)dim[1] &&
+                            (unsigned)(pos[2] + d[2]) < (unsigned)dim[2];
+            // the offset is masked, not selected: a select between two
+            // addresses becomes a select between two loads, one of them the
+            // load at q already made, and the other one a conditional load
+            return (uint32_t)((int)q + ((d[0] * sx + d[1] * sy + d[2]) & -(int)in));
+        };
+        const int tq = type[q];
+        // types of the rows q - e_o (nb even) and q + e_o (nb odd), o = nb / 2
+        int traw[6];
 #pragma unroll
-        for (int f = 0; f < 4; f++) acc[f] = inq ? alpha * tb.f[f][q] : tb.f[f][q];
-        if (inq) {
-            T l[4], xm[4], x0[4], xp[4];
+        for (int nb = 0; nb < 6; nb++) {
+            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
+            traw[nb] = type[at(o, k)];
+        }
+        const int bvq = bcv[q], btq = bct[q];
+        // node-value gradient and state-adjoint accumulator, read whatever the
+        // node types (their updates below are conditional)
+        T gbq[4], caq[4];
 #pragma unroll
-            for (int f = 0; f < 4; f++) {
-                l[f] = lam.f[f][q];
-                xm[f] = out.f[f][q - s];
-                x0[f] = out.f[f][q];
-                xp[f] = out.f[f][q + s];
+        for (int f = 0; f < 4; f++) {
+            gbq[f] = gb.f[f][q];
+            caq[f] = (zmode && !first_z) ? cacc.f[f][q] : (T)0;
+        }
+
+        // own row: lam, the sweep output around q, the T and velocity stencils
+        T tbq[4], l[4], xm[4], x0[4], xp[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            tbq[f] = tb.f[f][q];
+            l[f] = lam.f[f][q];
+            xm[f] = out.f[f][at(A, -1)];
+            x0[f] = out.f[f][q];
+            xp[f] = out.f[f][at(A, 1)];
+        }
+        const T tgrad = (Tt[at(A, 1)] - Tt[at(A, -1)]) * p.inv_2h;
+        T da[3], dA[3];  // D_a F_f and D_o F_a at q
+#pragma unroll
+        for (int f = 0; f < 3; f++) da[f] = (F[f][at(A, 1)] - F[f][at(A, -1)]) * ia;
+#pragma unroll
+        for (int o = 0; o < 3; o++)
+            if (o != A) dA[o] = (F[A][at(o, 1)] - F[A][at(o, -1)]) * io[o];
+
+        // neighbour rows r = q + k e_o: t_phi lam_T(r) and the differences
+        // their stencils need
+        T kr[6], lr[6], ra[6][3], rA[6][3];
+#pragma unroll
+        for (int nb = 0; nb < 6; nb++) {
+            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
+            const uint32_t r = at(o, k);
+            kr[nb] = p.t_phi * lam.f[3][r];
+            if (o == A) {
+                lr[nb] = lam.f[A][r];
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+                    ra[nb][f] = (F[f][at(A, k + 1)] - F[f][at(A, k - 1)]) * ia;
+#pragma unroll
+                for (int o2 = 0; o2 < 3; o2++)
+                    if (o2 != A) rA[nb][o2] = (F[A][at(A, k, o2, 1)] - F[A][at(A, k, o2, -1)]) * io[o2];
+            } else {
+                ra[nb][o] = (F[o][at(o, k, A, 1)] - F[o][at(o, k, A, -1)]) * ia;
             }
+        }
+        // every value above is used here, once, all together: a value used
+        // under a condition only would be loaded there, behind its own wait
+        asm volatile("" ::"v"(tq), "v"(bvq), "v"(btq));
+#pragma unroll
+        for (int nb = 0; nb < 6; nb++) asm volatile("" ::"v"(traw[nb]), "v"(kr[nb]));
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            asm volatile("" ::"v"(tbq[f]), "v"(l[f]), "v"(xm[f]), "v"(x0[f]), "v"(xp[f]),
+                         "v"(gbq[f]), "v"(caq[f]));
+        asm volatile("" ::"v"(tgrad));
+#pragma unroll
+        for (int f = 0; f < 3; f++) asm volatile("" ::"v"(da[f]));
+#pragma unroll
+        for (int o = 0; o < 3; o++)
+            if (o != A) asm volatile("" ::"v"(dA[o]));
+#pragma unroll
+        for (int nb = 0; nb < 6; nb++) {
+            const int o = nb >> 1;
+            if (o == A) {
+                asm volatile("" ::"v"(lr[nb]));
+#pragma unroll
+                for (int f = 0; f < 3; f++) asm volatile("" ::"v"(ra[nb][f]));
+#pragma unroll
+                for (int o2 = 0; o2 < 3; o2++)
+                    if (o2 != A) asm volatile("" ::"v"(rA[nb][o2]));
+            } else {
+                asm volatile("" ::"v"(ra[nb][o]));
+            }
+        }
+        const bool inq = tq == NODE_IN;
+        uint8_t tn[6];
+#pragma unroll
+        for (int nb = 0; nb < 6; nb++) {
+            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
+            tn[nb] = (unsigned)(pos[o] + k) < (unsigned)dim[o] ? (uint8_t)traw[nb] : kOutside;
+        }
+        const bool doxg = last && (tq == NODE_BOUND || tq == NODE_VALVE);
+
+        T acc[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) acc[f] = inq ? alpha * tbq[f] : tbq[f];
+        if (inq) {
             const T abv = -(l[0] * xm[0] + l[1] * xm[1] + l[2] * xm[2]);
             const T bbv = -(l[0] * x0[0] + l[1] * x0[1] + l[2] * x0[2]);
             const T cbv = -(l[0] * xp[0] + l[1] * xp[1] + l[2] * xp[2]);
             const T abt = -l[3] * xm[3], bbt = -l[3] * x0[3], cbt = -l[3] * xp[3];
-            acc[AXIS] += (-abv + cbv - abt + cbt) * p.inv_2h;
+            acc[A] += (-abv + cbv - abt + cbt) * p.inv_2h;
             part[1] += (double)((-abv - cbv + 2 * bbv) * inv_h2);
             part[2] += (double)((-abt - cbt + 2 * bbt) * inv_h2);
-            part[0] += (double)(-l[AXIS] * ((Tt[q + s] - Tt[q - s]) * p.inv_2h));
-            T phi;
-            if (AXIS == AX_X) {
-                T u_x = (U[q + sx] - U[q - sx]) * ix, v_x = (V[q + sx] - V[q - sx]) * ix;
-                T w_x = (W[q + sx] - W[q - sx]) * ix, u_y = (U[q + sy] - U[q - sy]) * iy;
-                T u_z = (U[q + 1] - U[q - 1]) * iz;
-                phi = 2 * u_x * u_x + v_x * v_x + w_x * w_x + v_x * u_y + w_x * u_z;
-            } else if (AXIS == AX_Y) {
-                T u_y = (U[q + sy] - U[q - sy]) * iy, v_y = (V[q + sy] - V[q - sy]) * iy;
-                T w_y = (W[q + sy] - W[q - sy]) * iy, v_x = (V[q + sx] - V[q - sx]) * ix;
-                T v_z = (V[q + 1] - V[q - 1]) * iz;
-                phi = u_y * u_y + 2 * v_y * v_y + w_y * w_y + u_y * v_x + w_y * v_z;
-            } else {
-                T u_z = (U[q + 1] - U[q - 1]) * iz, v_z = (V[q + 1] - V[q - 1]) * iz;
-                T w_z = (W[q + 1] - W[q - 1]) * iz, w_x = (W[q + sx] - W[q - sx]) * ix;
-                T w_y = (W[q + sy] - W[q - sy]) * iy;
-                phi = u_z * u_z + v_z * v_z + 2 * w_z * w_z + u_z * w_x + v_z * w_y;
-            }
+            part[0] += (double)(-l[A] * tgrad);
+            T phi = 0;
+#pragma unroll
+            for (int f = 0; f < 3; f++) phi += f == A ? 2 * da[f] * da[f] : da[f] * da[f];
+#pragma unroll
+            for (int o = 0; o < 3; o++)
+                if (o != A) phi += da[o] * dA[o];
             part[3] += (double)(l[3] * phi);
-#pragma unroll
-            for (int f = 0; f < 4; f++) {
-                const T ib = p.three_dt * l[f];
-                if (last) {
-                    // the state adjoint is completed below with tbar
-                    acc_state[f] = cacc.f[f][q] + ib;
-                } else if (zmode) {
-                    cacc.f[f][q] += ib;
-                    xb.f[f][q] = 0;
-                } else {
-                    xb.f[f][q] = ib;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 4; f++) xb.f[f][q] = 0;
         }
-        // transposed stencils of the rows that read node q: row c = q - e
-        // contributes with +, row c = q + e with -
+        // transposed stencils of the rows that read node q: row r = q - e
+        // contributes with +, row r = q + e with -
 #pragma unroll
         for (int nb = 0; nb < 6; nb++) {
-            const int e = nb >> 1;
-            const int sg = (nb & 1) ? 1 : -1;
-            const int pe = e == 0 ? x : e == 1 ? y : z;
-            const int de = e == 0 ? g.nx : e == 1 ? g.ny : g.nz;
-            const uint32_t se = e == 0 ? sx : e == 1 ? sy : 1u;
-            const int pc = pe + sg;
-            if (pc < 0 || pc >= de) continue;
-            const uint32_t c = sg < 0 ? q - se : q + se;
-            // the stencils below read around c: only a NODE_IN c, never on a
-            // grid face, keeps them inside the grid
-            if (type[c] != NODE_IN) continue;
-            const T fsg = sg < 0 ? (T)1 : (T)-1;
-            const T k = p.t_phi * lam.f[3][c];
-            if (AXIS == AX_X) {
-                if (e == 0) {
-                    const T u_x = (U[c + sx] - U[c - sx]) * ix, v_x = (V[c + sx] - V[c - sx]) * ix;
-                    const T w_x = (W[c + sx] - W[c - sx]) * ix, u_y = (U[c + sy] - U[c - sy]) * iy;
-                    const T u_z = (U[c + 1] - U[c - 1]) * iz;
-                    acc[0] += fsg * (k * 4 * u_x * ix);
-                    acc[1] += fsg * (k * (2 * v_x + u_y) * ix);
-                    acc[2] += fsg * (k * (2 * w_x + u_z) * ix);
-                    acc[3] += fsg * (-p.v_T * lam.f[0][c] * p.inv_2h);
-                } else if (e == 1) {
-                    acc[0] += fsg * (k * ((V[c + sx] - V[c - sx]) * ix) * iy);
-                } else {
-                    acc[0] += fsg * (k * ((W[c + sx] - W[c - sx]) * ix) * iz);
+            const int o = nb >> 1;
+            const T fsg = (nb & 1) ? (T)-1 : (T)1;
+            const T k = kr[nb];
+            const bool ok = tn[nb] == NODE_IN;
+            if (o == A) {
+#pragma unroll
+                for (int f = 0; f < 3; f++) {
+                    const T d = f == A ? k * 4 * ra[nb][f] * ia
+                                       : k * (2 * ra[nb][f] + rA[nb][f]) * ia;
+                    acc[f] += ok ? fsg * d : (T)0;
                 }
-            } else if (AXIS == AX_Y) {
-                if (e == 1) {
-                    const T u_y = (U[c + sy] - U[c - sy]) * iy, v_y = (V[c + sy] - V[c - sy]) * iy;
-                    const T w_y = (W[c + sy] - W[c - sy]) * iy, v_x = (V[c + sx] - V[c - sx]) * ix;
-                    const T v_z = (V[c + 1] - V[c - 1]) * iz;
-                    acc[0] += fsg * (k * (2 * u_y + v_x) * iy);
-                    acc[1] += fsg * (k * 4 * v_y * iy);
-                    acc[2] += fsg * (k * (2 * w_y + v_z) * iy);
-                    acc[3] += fsg * (-p.v_T * lam.f[1][c] * p.inv_2h);
-                } else if (e == 0) {
-                    acc[1] += fsg * (k * ((U[c + sy] - U[c - sy]) * iy) * ix);
-                } else {
-                    acc[1] += fsg * (k * ((W[c + sy] - W[c - sy]) * iy) * iz);
-                }
+                acc[3] += ok ? fsg * (-p.v_T * lr[nb] * p.inv_2h) : (T)0;
             } else {
-                if (e == 2) {
-                    const T u_z = (U[c + 1] - U[c - 1]) * iz, v_z = (V[c + 1] - V[c - 1]) * iz;
-                    const T w_z = (W[c + 1] - W[c - 1]) * iz, w_x = (W[c + sx] - W[c - sx]) * ix;
-                    const T w_y = (W[c + sy] - W[c - sy]) * iy;
-                    acc[0] += fsg * (k * (2 * u_z + w_x) * iz);
-                    acc[1] += fsg * (k * (2 * v_z + w_y) * iz);
-                    acc[2] += fsg * (k * 4 * w_z * iz);
-                    acc[3] += fsg * (-p.v_T * lam.f[2][c] * p.inv_2h);
-                } else if (e == 0) {
-                    acc[2] += fsg * (k * ((U[c + 1] - U[c - 1]) * iz) * ix);
-                } else {
-                    acc[2] += fsg * (k * ((V[c + 1] - V[c - 1]) * iz) * iy);
-                }
+                acc[A] += ok ? fsg * (k * ra[nb][o] * io[o]) : (T)0;
             }
         }
-        if (last) {
-            // end of the reversed step (the Picard layer started as the state,
-            // whose BOUND/VALVE nodes UpdateBoundaries filled with node values):
-            // xbar of the previous step, node-value gradients, cleared accumulators
+
+        // node-value gradient: lam on the sweep's boundary rows whose RHS is the
+        // node value (both matrices separately), and at the end of a reversed
+        // step tbar on BOUND/VALVE nodes
+        const int rl = row_role((uint8_t)tq, tn[2 * A], tn[2 * A + 1]);
+        const bool bnd = rl != RL_IN && rl != RL_NONE;
+        const bool ends = rl == RL_START || rl == RL_END;
+        const bool gv = bnd && (!ends || bvq != BC_FREE), gt = bnd && (!ends || btq != BC_FREE);
 #pragma unroll
-            for (int f = 0; f < 4; f++) {
-                if (inq) {
-                    xb.f[f][q] = acc_state[f] + acc[f];
-                    cacc.f[f][q] = 0;
-                } else if (tq == NODE_BOUND || tq == NODE_VALVE) {
-                    gb.f[f][q] += acc[f];
-                }
-                tb.f[f][q] = 0;
+        for (int f = 0; f < 4; f++) {
+            const bool gl = f < 3 ? gv : gt;
+            if (gl || doxg) gb.f[f][q] = (gbq[f] + (gl ? l[f] : (T)0)) + (doxg ? acc[f] : (T)0);
+        }
+
+        // stores: tbar; a Z sweep's input adjoint goes to the state adjoint's
+        // accumulator (the first Z sweep of a reversed step starts it)
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const T ib = p.three_dt * l[f];
+            if (zmode && inq) {
+                const T sa = first_z ? ib : caq[f] + ib;
+                if (last) xb.f[f][q] = sa + acc[f];
+                else cacc.f[f][q] = sa;
+            } else if (last) {
+                xb.f[f][q] = 0;
             }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 4; f++) tb.f[f][q] = acc[f];
+            // end of the reversed step (the Picard layer started as the
+            // state, whose BOUND/VALVE nodes UpdateBoundaries filled with
+            // node values): xbar of the previous step, tbar cleared
+            tb.f[f][q] = last ? (T)0 : acc[f];
         }
     }
     // block sums of the parameter partials, added to this block's slot
@@ -499,16 +579,19 @@ int adj_waves(int n, int lpg) {
 
 }  // namespace
 
+// Lines per tile of the transposed X/Y solves: 128-byte rows (32 FP32 lines),
+// or 64-byte rows (16 lines) where a line is longer than 128 nodes, so that a
+// work-group holds at most 8 waves (the register budget the solve needs).
 template <class T>
-int adj_lpg() {
-    return 128 / (int)sizeof(T);
+int adj_lpg(const Geom& g) {
+    return (sizeof(T) == 4 && std::max(g.nx, g.ny) <= 128) ? 32 : 16;
 }
 
 template <class T>
 bool adjoint_supported(const Geom& g) {
-    const int lpg = adj_lpg<T>();
+    const int lpg = adj_lpg<T>(g);
     for (int n : {g.nx, g.ny})
-        if (adj_waves(n, lpg) * 64 > 32 * lpg) return false;
+        if (adj_waves(n, lpg) > 8) return false;
     return adj_z_chunk(g.nz) != 0;
 }
 
@@ -565,13 +648,39 @@ void build_adj_tiles(const uint8_t* t, const Geom& g, int axis, int lpg, int32_t
 
 int adj_point_blocks() { return kPointBlocks; }
 
+namespace {
+
+// Grid of adj_point: the blocks that are resident at once on this device at
+// the kernel's occupancy (one round, no tail), at most kPointBlocks.
+template <class T, int AX>
+unsigned point_grid() {
+    static std::atomic<int> cached[16];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return kPointBlocks;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n == 0) {
+        int per = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, adj_point_kernel<T, AX>, 256, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per <= 0 || cus <= 0)
+            n = kPointBlocks;
+        else
+            n = std::min(kPointBlocks, per * cus);
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return (unsigned)n;
+}
+
+}  // namespace
+
 template <class T>
 void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const uint8_t* type,
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
-                      Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc, Layer4<T> lam, Layer4<T> gb,
-                      T alpha, T beta, int zmode, int last, const int32_t* tiles, int64_t ntiles,
-                      int ltiles, double* pp, hipStream_t st) {
-    constexpr int LPG = 128 / (int)sizeof(T);
+                      Layer4<T> xs, T xscale, Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc,
+                      Layer4<T> lam, Layer4<T> gb, T alpha, T beta, int zmode, int first_z, int last,
+                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp, hipStream_t st) {
+    const int lpg = adj_lpg<T>(g);
     if (axis == AX_Z) {
         if (ntiles > 0) {
             const int m = adj_z_chunk(g.nz);
@@ -579,7 +688,7 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
             const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
 #define LZ(MM)                                                                                  \
     hipLaunchKernelGGL((adj_z_kernel<T, MM>), dim3((unsigned)ntiles), dim3(64 * kZWaves), lds, st, \
-                       g, p, type, bcv, bct, tin, xb, tb, beta, lam, gb, tiles, vec_ok)
+                       g, p, type, bcv, bct, tin, xs, xscale, tb, beta, lam, tiles, vec_ok)
             if (m == 2) LZ(2);
             else if (m == 4) LZ(4);
             else LZ(8);
@@ -588,24 +697,31 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
         }
     } else if (ntiles > 0) {
         const int n = axis == AX_X ? g.nx : g.ny;
-        const int waves = adj_waves(n, LPG);
-        const int P = waves * (64 / LPG);
-        const size_t lds = (size_t)LPG * (kRedFields * 2 * P + 1) * sizeof(T);
-#define LA(AX)                                                                                   \
-    hipLaunchKernelGGL((adj_solve_kernel<T, AX, kAdjChunk, LPG, 32 * LPG>), dim3((unsigned)ntiles), \
-                       dim3(64 * waves), lds, st, g, p, type, bcv, bct, tin, xb, tb, beta, lam, gb, \
+        const int waves = adj_waves(n, lpg);  // at most 8 (adjoint_supported)
+        const int P = waves * (64 / lpg);
+        const size_t lds = (size_t)lpg * (kRedFields * 2 * P + 1) * sizeof(T);
+#define LA(AX, LPG)                                                                               \
+    hipLaunchKernelGGL((adj_solve_kernel<T, AX, kAdjChunk, LPG, 512>), dim3((unsigned)ntiles),    \
+                       dim3(64 * waves), lds, st, g, p, type, bcv, bct, tin, xs, xscale, tb, beta, lam, \
                        tiles, ltiles)
-        if (axis == AX_X) LA(AX_X);
-        else LA(AX_Y);
+        if (lpg == 32) {
+            if (axis == AX_X) LA(AX_X, 32);
+            else LA(AX_Y, 32);
+        } else {
+            if (axis == AX_X) LA(AX_X, 16);
+            else LA(AX_Y, 16);
+        }
 #undef LA
         CMCX_LAUNCH_CHECK();
     }
     const T h = axis == AX_X ? p.hx : axis == AX_Y ? p.hy : p.hz;
     const T inv_h2 = (T)1 / (h * h);
-    // every launch uses the full fixed grid, so that slot b always belongs to block b
+    // every launch of one kernel uses the same grid, so that slot b always belongs
+    // to block b (slots past the grid stay zero)
 #define LP(AX)                                                                                  \
-    hipLaunchKernelGGL((adj_point_kernel<T, AX>), dim3(kPointBlocks), dim3(256), 0, st, g, p,  \
-                       inv_h2, type, tin, out, lam, tb, xb, cacc, gb, zmode, last, alpha, pp)
+    hipLaunchKernelGGL((adj_point_kernel<T, AX>), dim3(point_grid<T, AX>()), dim3(256), 0, st, g, p, \
+                       inv_h2, type, bcv, bct, tin, out, lam, tb, xb, cacc, gb, zmode, first_z, last, \
+                       alpha, pp)
     if (axis == AX_X) LP(AX_X);
     else if (axis == AX_Y) LP(AX_Y);
     else LP(AX_Z);
@@ -614,13 +730,13 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
 }
 
 #define CMCX_ADJ_INST(T)                                                                          \
-    template int adj_lpg<T>();                                                                   \
+    template int adj_lpg<T>(const Geom&);                                                        \
     template bool adjoint_supported<T>(const Geom&);                                             \
     template void launch_adj_sweep<T>(const Geom&, int, const SweepParams<T>&, const uint8_t*,    \
                                       const uint8_t*, const uint8_t*, Layer4<T>, Layer4<T>,       \
-                                      Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>, T, T, \
-                                      int, int, const int32_t*, int64_t, int, double*,             \
-                                      hipStream_t);
+                                      Layer4<T>, T, Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>,   \
+                                      Layer4<T>, T, T, int, int, int, const int32_t*, int64_t,    \
+                                      int, double*, hipStream_t);
 CMCX_ADJ_INST(float)
 CMCX_ADJ_INST(double)
 
