@@ -370,18 +370,18 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
     for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
         const int ux = (int)(q / (uint32_t)sx), rem = (int)q - ux * sx, uy = rem / sy;
         const int pos[3] = {ux, uy, rem - uy * sy};
-        // q + k1 e_o1 + k2 e_o2 inside the grid, else q
+        // q + k1 e_o1 + k2 e_o2 inside the grid, else q (only the axes that
+        // move are checked; the offset is masked, not selected: a select
+        // between two addresses becomes a select between two loads, one of
+        // them the load at q already made, and the other a conditional load)
+        auto inr = [&](int o, int k) { return (unsigned)(pos[o] + k) < (unsigned)dim[o]; };
         auto at = [&](int o1, int k1, int o2 = 0, int k2 = 0) -> uint32_t {
-            int d[3] = {0, 0, 0};
-            d[o1] += k1;
-            d[o2] += k2;
-            const bool in = (unsigned)(pos[0] + d[0]) < (unsigned)dim[0] &&
-                            (unsigned)(pos[1] + d[1]) < (unsigned)dim[1] &&
-                            (unsigned)(pos[2] + d[2]) < (unsigned)dim[2];
-            // the offset is masked, not selected: a select between two
-            // addresses becomes a select between two loads, one of them the
-            // load at q already made, and the other one a conditional load
-            return (uint32_t)((int)q + ((d[0] * sx + d[1] * sy + d[2]) & -(int)in));
+            bool in;
+            if (k2 == 0) in = k1 == 0 || inr(o1, k1);
+            else if (o1 == o2) in = k1 + k2 == 0 || inr(o1, k1 + k2);
+            else in = (k1 == 0 || inr(o1, k1)) && inr(o2, k2);
+            const int off = k1 * so[o1] + k2 * so[o2];
+            return (uint32_t)((int)q + (off & -(int)in));
         };
         const int tq = type[q];
         // types of the rows q - e_o (nb even) and q + e_o (nb odd), o = nb / 2
@@ -579,12 +579,15 @@ int adj_waves(int n, int lpg) {
 
 }  // namespace
 
-// Lines per tile of the transposed X/Y solves: 128-byte rows (32 FP32 lines),
-// or 64-byte rows (16 lines) where a line is longer than 128 nodes, so that a
-// work-group holds at most 8 waves (the register budget the solve needs).
+// Lines per tile of the transposed X/Y solves: 16 (64-byte FP32 rows).  A
+// work-group then holds a line's rows in ceil(n / 32) waves (3 for 96 nodes):
+// with the ~150 registers the solve takes, a SIMD holds 3 waves, and
+// work-groups of 6 waves (32 lines) run one per CU (their waves take two
+// slots on two SIMDs), so a grid of more tiles than CUs ran in two rounds;
+// 3-wave work-groups run 2-3 per CU, all tiles of a 96^3 grid at once.
 template <class T>
-int adj_lpg(const Geom& g) {
-    return (sizeof(T) == 4 && std::max(g.nx, g.ny) <= 128) ? 32 : 16;
+int adj_lpg(const Geom&) {
+    return 16;
 }
 
 template <class T>
@@ -704,13 +707,8 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
     hipLaunchKernelGGL((adj_solve_kernel<T, AX, kAdjChunk, LPG, 512>), dim3((unsigned)ntiles),    \
                        dim3(64 * waves), lds, st, g, p, type, bcv, bct, tin, xs, xscale, tb, beta, lam, \
                        tiles, ltiles)
-        if (lpg == 32) {
-            if (axis == AX_X) LA(AX_X, 32);
-            else LA(AX_Y, 32);
-        } else {
-            if (axis == AX_X) LA(AX_X, 16);
-            else LA(AX_Y, 16);
-        }
+        if (axis == AX_X) LA(AX_X, 16);
+        else LA(AX_Y, 16);
 #undef LA
         CMCX_LAUNCH_CHECK();
     }
