@@ -327,9 +327,11 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
 }
 
 // Pointwise part of a reversed sweep (see the file comment).  A fixed grid, so
-// that block b always owns partial-sum slot b; each block takes a contiguous
-// run of nodes, and the blocks of one XCD (b mod 8) take neighbouring runs, so
-// the stencil neighbours (x-planes either side) are mostly in that XCD's L2.
+// that block b always owns partial-sum slot b.  Each XCD (blocks b mod 8)
+// takes one contiguous eighth of the nodes and its blocks sweep it together,
+// one window of (blocks per XCD) x 256 consecutive nodes after another, so
+// that the stencil neighbours a block reads (x-planes either side) are read by
+// the XCD's other blocks at about the same time and stay in that XCD's L2.
 // Every load of a node goes out in one batch: an address is q + an offset of
 // at most two nodes per axis, replaced by q itself when it leaves the grid (so
 // no address waits for a node type), and the terms that the node types rule
@@ -364,10 +366,11 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
     const T* Tt = tin.f[3];
     double part[4] = {0, 0, 0, 0};
     const uint32_t N = (uint32_t)g.N;
-    const uint32_t per = ((N + gridDim.x - 1) / gridDim.x + 63) & ~63u;
-    const uint32_t q0 = xcd_remap(blockIdx.x, gridDim.x) * per;
-    const uint32_t q1 = min(N, q0 + per);
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+    // the grid is a multiple of 8 blocks (the launch sizes it so)
+    const uint32_t nbx = gridDim.x / 8, xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+    const uint32_t slab = ((N + 7) / 8 + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const uint32_t q1 = min(N, (xcd + 1) * slab);
+    for (uint32_t q = xcd * slab + local * blockDim.x + threadIdx.x; q < q1; q += nbx * blockDim.x) {
         const int ux = (int)(q / (uint32_t)sx), rem = (int)q - ux * sx, uy = rem / sy;
         const int pos[3] = {ux, uy, rem - uy * sy};
         // q + k1 e_o1 + k2 e_o2 inside the grid, else q (only the axes that
@@ -669,7 +672,7 @@ unsigned point_grid() {
             per <= 0 || cus <= 0)
             n = kPointBlocks;
         else
-            n = std::min(kPointBlocks, per * cus);
+            n = std::max(8, std::min(kPointBlocks, per * cus) / 8 * 8);
         cached[dev].store(n, std::memory_order_relaxed);
     }
     return (unsigned)n;
