@@ -212,16 +212,65 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
     }
 }
 
+// Loads of one lane's run of M rows at c0 (the first `valid` of them inside
+// the line), all issued whatever `valid` is: rows past it read from node 0
+// (addresses masked, not selected, so that no load becomes conditional).
+// `vall`: every run of the launch is whole and 16-byte aligned (vector loads).
+template <class T, int M, bool VALL>
+__device__ __forceinline__ void load_run_raw(const T* __restrict__ p, int64_t c0, int valid, T* r) {
+    constexpr int VW = Vec16<T>::n;
+    if constexpr (M % VW == 0 && VALL) {
+        {
+            using V = typename Vec16<T>::type;
+            const V* q = reinterpret_cast<const V*>(p + (c0 & -(int64_t)(valid > 0)));
+#pragma unroll
+            for (int k = 0; k < M / VW; k++) {
+                const V v = q[k];
+                const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+                for (int i = 0; i < VW; i++) r[k * VW + i] = e[i];
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) r[j] = p[(c0 + j) & -(int64_t)(j < valid)];
+}
+
+template <int M, bool VALL>
+__device__ __forceinline__ void load_bytes_raw(const uint8_t* __restrict__ p, int64_t c0, int valid,
+                                               int* r) {
+    if constexpr ((M == 4 || M == 8) && VALL) {
+        {
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(p + (c0 & -(int64_t)(valid > 0)));
+#pragma unroll
+            for (int k = 0; k < M / 4; k++) {
+                const uint32_t v = q[k];
+#pragma unroll
+                for (int i = 0; i < 4; i++) r[4 * k + i] = (v >> (8 * i)) & 0xff;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) r[j] = p[(c0 + j) & -(int64_t)(j < valid)];
+}
+
+template <class T>
+__device__ __forceinline__ void pin(const T& v) {
+    asm volatile("" ::"v"(v));
+}
+
 // Transposed line solve along z in the contiguous layout of the forward Z
 // sweep: 32 lanes per line, lane l owns rows [l*M, l*M+M) (16-byte vector
 // accesses), two lines per wavefront, kZWaves wavefronts per work-group.  The
 // coefficients of the rows either side of a chunk come from the neighbouring
 // lanes by shuffle (c of the previous row, a of the next row).
-template <class T, int M>
+template <class T, int M, bool VALL>
 __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     Geom g, SweepParams<T> p, const uint8_t* __restrict__ type, const uint8_t* __restrict__ bcv,
     const uint8_t* __restrict__ bct, Layer4<T> tin, Layer4<T> xs, T xscale, Layer4<T> tb, T beta,
-    Layer4<T> lam, const int32_t* __restrict__ lines, int vec_ok) {
+    Layer4<T> lam, Layer4<T> cacc, int zacc, const int32_t* __restrict__ lines) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
@@ -235,12 +284,47 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     const int r0 = ll * M;
     const bool ok = live && ll < used;
     const int valid = ok ? min(M, n - r0) : 0;
-    const bool vec = vec_ok && valid == M;
+    const bool vec = VALL && valid == M;
     const int64_t c0 = base + r0;
+
+    // Every load of the lane goes out in one batch (a value used under a
+    // condition only would be loaded there, behind its own wait): types, BC
+    // kinds, the advecting velocity, xbar, tbar and the accumulator of the
+    // lane's rows; the raw values are then used once, all together.
+    int tyr[M], bvr[M], btr[M];
+    load_bytes_raw<M, VALL>(type, c0, valid, tyr);
+    load_bytes_raw<M, VALL>(bcv, c0, valid, bvr);
+    load_bytes_raw<M, VALL>(bct, c0, valid, btr);
+    T w[M];
+    load_run_raw<T, M, VALL>(tin.f[2], c0, valid, w);
+    const bool hasx = xs.f[0] != nullptr;  // xbar = xscale * xs, or zero
+    T xa[4][M], ta[4][M], ca[4][M];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        load_run_raw<T, M, VALL>(hasx ? xs.f[v] : tb.f[v], c0, valid, xa[v]);
+        load_run_raw<T, M, VALL>(tb.f[v], c0, valid, ta[v]);
+        // the state adjoint's accumulator (the input of a Z sweep is the
+        // state): (3/dt) lam is added to it on NODE_IN rows (zacc 2) or
+        // starts it (zacc 1)
+        load_run_raw<T, M, VALL>(zacc == 2 ? cacc.f[v] : tb.f[v], c0, valid, ca[v]);
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        pin(tyr[j]);
+        pin(bvr[j]);
+        pin(btr[j]);
+        pin(w[j]);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            pin(xa[v][j]);
+            pin(ta[v][j]);
+            pin(ca[v][j]);
+        }
+    }
 
     uint8_t ty[M + 2];
 #pragma unroll
-    for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? type[c0 + j] : kOutside;
+    for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? (uint8_t)tyr[j] : kOutside;
     {
         const int prev = __shfl((int)ty[M], lane - 1, 64);
         const int nxt = __shfl((int)ty[1], lane + 1, 64);
@@ -250,16 +334,11 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     int role[M];
 #pragma unroll
     for (int j = 0; j < M; j++) role[j] = row_role(ty[j + 1], ty[j], ty[j + 2]);
-    T w[M];
-    load_run<T, M>(tin.f[2] + c0, w, vec, valid);
     Coef<T> k[M];
 #pragma unroll
-    for (int j = 0; j < M; j++) {
-        const int64_t c = j < valid ? c0 + j : 0;
-        const uint8_t bv = bcv[c], bt = bct[c];
-        k[j] = row_coef<T>(role[j], bv == BC_FREE, bt == BC_FREE,
+    for (int j = 0; j < M; j++)
+        k[j] = row_coef<T>(role[j], bvr[j] == BC_FREE, btr[j] == BC_FREE,
                            role[j] == RL_IN ? w[j] * p.inv_2h : (T)0, p);
-    }
     // c of row r0-1 and a of row r0+M (zero outside the line: identity rows)
     T cvm = __shfl(k[M - 1].cv, lane - 1, 64), ctm = __shfl(k[M - 1].ct, lane - 1, 64);
     T avp = __shfl(k[0].av, lane + 1, 64), atp = __shfl(k[0].at, lane + 1, 64);
@@ -268,20 +347,13 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
 
     T rhs[4][M];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        T xa[M], ta[M];
-        if (xs.f[0] != nullptr) {  // xbar = xscale * xs, or zero
-            load_run<T, M>(xs.f[v] + c0, xa, vec, valid);
+    for (int v = 0; v < 4; v++)
 #pragma unroll
-            for (int j = 0; j < M; j++) xa[j] *= xscale;
-        } else {
-#pragma unroll
-            for (int j = 0; j < M; j++) xa[j] = 0;
+        for (int j = 0; j < M; j++) {
+            const T xv = hasx ? xscale * xa[v][j] : (T)0;
+            rhs[v][j] = role[j] == RL_IN ? xv + beta * ta[v][j] : (T)0;
+            if (zacc != 2) ca[v][j] = 0;
         }
-        load_run<T, M>(tb.f[v] + c0, ta, vec, valid);
-#pragma unroll
-        for (int j = 0; j < M; j++) rhs[v][j] = role[j] == RL_IN ? xa[j] + beta * ta[j] : (T)0;
-    }
     Chunk<T, M> ch;
     Row<T> rfirst, rlast;
 #pragma unroll
@@ -323,6 +395,12 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
             }
         }
         store_run<T, M>(lam.f[v] + c0, x, vec, valid);
+        if (zacc) {
+#pragma unroll
+            for (int j = 0; j < M; j++)
+                if (role[j] == RL_IN) ca[v][j] = zacc == 2 ? ca[v][j] + p.three_dt * x[j] : p.three_dt * x[j];
+            store_run<T, M>(cacc.f[v] + c0, ca[v], vec, valid);
+        }
     }
 }
 
@@ -353,8 +431,8 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
                                                         const uint8_t* __restrict__ bct,
                                                         Layer4<T> tin, Layer4<T> out, Layer4<T> lam,
                                                         Layer4<T> tb, Layer4<T> xb, Layer4<T> cacc,
-                                                        Layer4<T> gb, int zmode, int first_z,
-                                                        int last, T alpha, double* __restrict__ pp) {
+                                                        Layer4<T> gb, int last, T alpha,
+                                                        double* __restrict__ pp) {
     constexpr int A = AXIS;
     // a grid holds fewer than 2^31 nodes (checked at creation): 32-bit indices
     const int sx = (int)g.sx, sy = (int)g.sy;
@@ -401,7 +479,7 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             gbq[f] = gb.f[f][q];
-            caq[f] = (zmode && !first_z) ? cacc.f[f][q] : (T)0;
+            caq[f] = last ? cacc.f[f][q] : (T)0;
         }
 
         // own row: lam, the sweep output around q, the T and velocity stencils
@@ -534,18 +612,11 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
             if (gl || doxg) gb.f[f][q] = (gbq[f] + (gl ? l[f] : (T)0)) + (doxg ? acc[f] : (T)0);
         }
 
-        // stores: tbar; a Z sweep's input adjoint goes to the state adjoint's
-        // accumulator (the first Z sweep of a reversed step starts it)
+        // stores: tbar (the input adjoint of an X or Y sweep stays in lam for the
+        // next solve; that of a Z sweep went to the accumulator in its solve)
 #pragma unroll
         for (int f = 0; f < 4; f++) {
-            const T ib = p.three_dt * l[f];
-            if (zmode && inq) {
-                const T sa = first_z ? ib : caq[f] + ib;
-                if (last) xb.f[f][q] = sa + acc[f];
-                else cacc.f[f][q] = sa;
-            } else if (last) {
-                xb.f[f][q] = 0;
-            }
+            if (last) xb.f[f][q] = inq ? caq[f] + acc[f] : (T)0;
             // end of the reversed step (the Picard layer started as the
             // state, whose BOUND/VALVE nodes UpdateBoundaries filled with
             // node values): xbar of the previous step, tbar cleared
@@ -691,13 +762,21 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
         if (ntiles > 0) {
             const int m = adj_z_chunk(g.nz);
             const size_t lds = 0;
-            const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 ? 1 : 0;
-#define LZ(MM)                                                                                  \
-    hipLaunchKernelGGL((adj_z_kernel<T, MM>), dim3((unsigned)ntiles), dim3(64 * kZWaves), lds, st, \
-                       g, p, type, bcv, bct, tin, xs, xscale, tb, beta, lam, tiles, vec_ok)
-            if (m == 2) LZ(2);
-            else if (m == 4) LZ(4);
-            else LZ(8);
+            // whole 16-byte aligned runs: every lane's M rows inside its line
+            const int vec_ok = ((size_t)g.nz * sizeof(T)) % 16 == 0 && g.nz % m == 0 ? 1 : 0;
+#define LZ(MM, VA)                                                                              \
+    hipLaunchKernelGGL((adj_z_kernel<T, MM, VA>), dim3((unsigned)ntiles), dim3(64 * kZWaves), lds, \
+                       st, g, p, type, bcv, bct, tin, xs, xscale, tb, beta, lam, cacc,             \
+                       zmode ? (first_z ? 1 : 2) : 0, tiles)
+            if (vec_ok) {
+                if (m == 2) LZ(2, true);
+                else if (m == 4) LZ(4, true);
+                else LZ(8, true);
+            } else {
+                if (m == 2) LZ(2, false);
+                else if (m == 4) LZ(4, false);
+                else LZ(8, false);
+            }
 #undef LZ
             CMCX_LAUNCH_CHECK();
         }
@@ -721,8 +800,7 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
     // to block b (slots past the grid stay zero)
 #define LP(AX)                                                                                  \
     hipLaunchKernelGGL((adj_point_kernel<T, AX>), dim3(point_grid<T, AX>()), dim3(256), 0, st, g, p, \
-                       inv_h2, type, bcv, bct, tin, out, lam, tb, xb, cacc, gb, zmode, first_z, last, \
-                       alpha, pp)
+                       inv_h2, type, bcv, bct, tin, out, lam, tb, xb, cacc, gb, last, alpha, pp)
     if (axis == AX_X) LP(AX_X);
     else if (axis == AX_Y) LP(AX_Y);
     else LP(AX_Z);
