@@ -1,14 +1,16 @@
-"""The in-bounds invariant of the adjoint's pointwise gather (adj_point_kernel
-in cmc-fluid-solver_amd/csrc/kernels_adjoint.hip), checked on the CPU.
+"""The addressing of the adjoint's pointwise gather (adj_point_kernel in
+cmc-fluid-solver_amd/csrc/kernels_adjoint.hip), checked on the CPU.
 
-For node q the kernel reads, when q is NODE_IN, q +- s (the sweep axis) and
-the stencil nodes q +- sx, q +- sy, q +- 1; and for each of q's six grid
-neighbours c that is NODE_IN (its test on type[c] comes before those loads),
-the stencil nodes c +- sx, c +- sy, c +- 1.  On a mask that cmcx_create
-accepts (no NODE_IN node on a box face) every NODE_IN node has all three
-coordinates in [1, n-2], so each of those flat indices is inside the grid and
-is the intended neighbour (no wrap into the next line).  The restatement below
-forms the same index sets with coordinates and flat arithmetic side by side.
+For sweep axis a and node q the kernel reads, whatever the node types (all
+loads go out in one batch), q + k1 e_o1 + k2 e_o2 for the offsets listed in
+`kernel_offsets`; an offset is kept only when every axis it moves along stays
+inside the grid (checked against q's own coordinates), else the load reads q.
+So every address is inside the grid.  The values the kernel then uses are
+those of q's own stencil when q is NODE_IN and those of row r = q + k e_o's
+stencil when r is NODE_IN; on a mask that cmcx_create accepts (no NODE_IN node
+on a box face) every such offset is kept, i.e. the load reads the intended
+neighbour and never a clamped stand-in.  The restatement below forms the same
+addresses with flat arithmetic beside the coordinates.
 """
 import numpy as np
 import pytest
@@ -31,35 +33,77 @@ def mask_accepted(t):
     return True
 
 
-def gather_indices(t):
-    """Coordinates of every node the gather reads, as (x, y, z) arrays (may be
-    out of range), and the flat indices formed for them."""
+def kernel_offsets(a):
+    """(offset, user) pairs of the kernel for sweep axis a: user None = always
+    used (neighbour types and lam_T of the neighbour rows), "q" = used when q
+    is NODE_IN, (o, k) = used when row q + k e_o is NODE_IN."""
+    def vec(*pairs):
+        d = [0, 0, 0]
+        for o, k in pairs:
+            d[o] += k
+        return tuple(d)
+    offs = []
+    for o in range(3):
+        for k in (-1, 1):
+            offs.append((vec((o, k)), None))          # type, lam_T, lam_a of the row
+    for k in (-1, 1):
+        offs.append((vec((a, k)), "q"))               # sweep output, T, velocity along a
+    for o in range(3):
+        if o != a:
+            for k in (-1, 1):
+                offs.append((vec((o, k)), "q"))       # D_o F_a at q
+    for o in range(3):
+        for k in (-1, 1):
+            if o == a:
+                offs.append((vec((a, k + 1)), (o, k)))   # D_a F_f at the row
+                offs.append((vec((a, k - 1)), (o, k)))
+                for o2 in range(3):
+                    if o2 != a:
+                        offs.append((vec((a, k), (o2, 1)), (o, k)))   # D_o2 F_a at the row
+                        offs.append((vec((a, k), (o2, -1)), (o, k)))
+            else:
+                offs.append((vec((o, k), (a, 1)), (o, k)))    # D_a F_o at the row
+                offs.append((vec((o, k), (a, -1)), (o, k)))
+    return offs
+
+
+def _check(t, a):
+    """True when every address of the gather along axis a is in the grid and
+    every used stencil value is the intended neighbour's."""
     nx, ny, nz = t.shape
+    dims = np.array([nx, ny, nz])
     sx, sy = ny * nz, nz
-    fin = t == domains.NODE_IN
-    xs, ys, zs = np.nonzero(fin)
-    offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
-    coords, flats = [], []
-    # stencils around a NODE_IN node: the node's own (q) and those of the rows
-    # c that read q (c a NODE_IN neighbour of q) are both "stencil of a NODE_IN
-    # node", so one set covers both
-    for dx, dy, dz in offs:
-        X, Y, Z = xs + dx, ys + dy, zs + dz
-        coords.append((X, Y, Z))
-        flats.append((xs * sx + ys * sy + zs) + dx * sx + dy * sy + dz)
-    return coords, flats, (nx, ny, nz)
-
-
-def _check(t):
-    coords, flats, (nx, ny, nz) = gather_indices(t)
     N = nx * ny * nz
-    for (X, Y, Z), F in zip(coords, flats):
-        inside = (X >= 0) & (X < nx) & (Y >= 0) & (Y < ny) & (Z >= 0) & (Z < nz)
-        if not inside.all():
-            return False
-        assert np.all((F >= 0) & (F < N))
-        assert np.array_equal(F, (X * ny + Y) * nz + Z)
-    return True
+    X, Y, Z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = [X.ravel(), Y.ravel(), Z.ravel()]
+    q = (pos[0] * sx + pos[1] * sy + pos[2]).astype(np.int64)
+    tf = t.ravel()
+    fin = tf == domains.NODE_IN
+    ok = True
+    for d, user in kernel_offsets(a):
+        keep = np.ones(N, bool)
+        for o in range(3):
+            if d[o]:
+                keep &= (pos[o] + d[o] >= 0) & (pos[o] + d[o] < dims[o])
+        off = d[0] * sx + d[1] * sy + d[2]
+        addr = q + np.where(keep, off, 0)           # the kernel's masked offset
+        assert np.all((addr >= 0) & (addr < N))     # every load inside the grid
+        if user is None:
+            continue
+        if user == "q":
+            used = fin
+        else:
+            o, k = user
+            inr = (pos[o] + k >= 0) & (pos[o] + k < dims[o])
+            r = q + np.where(inr, k * (sx, sy, 1)[o], 0)
+            used = inr & fin[r]
+        if not np.all(keep[used]):
+            ok = False
+            continue
+        # a kept offset is the neighbour by coordinates (no wrap into another line)
+        tgt = ((pos[0] + d[0]) * ny + (pos[1] + d[1])) * nz + (pos[2] + d[2])
+        assert np.array_equal(addr[used], tgt[used])
+    return ok
 
 
 @pytest.mark.parametrize("name", ["ellipsoids64", "lid_sphere32", "random", "channel48", "batch"])
@@ -78,18 +122,33 @@ def test_gather_in_bounds_on_accepted_masks(name):
     for d in ds:
         t = d.type.reshape(d.shape)
         assert mask_accepted(t)
-        assert _check(t)
+        for a in range(3):
+            assert _check(t, a)
 
 
-def test_face_node_would_leave_the_grid():
-    """The invariant needs the creation check: a NODE_IN node on a face (a mask
-    cmcx_create rejects with CMCX_ERR_MASK) makes the gather form an index
-    outside the grid or a wrapped neighbour."""
+def test_offsets_cover_the_stencils():
+    """The offset list reaches what the transposed stencils need: two nodes
+    along the sweep axis, the diagonal neighbours of the rows along a, and
+    nothing further than one node off the sweep axis."""
+    for a in range(3):
+        ds = {d for d, _ in kernel_offsets(a)}
+        assert tuple(2 if o == a else 0 for o in range(3)) in ds
+        assert tuple(-2 if o == a else 0 for o in range(3)) in ds
+        for d in ds:
+            assert all(abs(d[o]) <= (2 if o == a else 1) for o in range(3))
+            assert sum(1 for o in range(3) if d[o]) <= 2
+
+
+def test_face_node_would_use_a_stand_in():
+    """The invariant needs the creation check: with a NODE_IN node on a face
+    (a mask cmcx_create rejects with CMCX_ERR_MASK) some used stencil offset
+    leaves the grid, so the kernel would read the clamped stand-in q; the
+    addresses themselves still stay inside the grid."""
     d = domains.box(8, 9, 10)
     t = d.type.reshape(d.shape).copy()
     t[4, 4, 0] = domains.NODE_IN
     assert not mask_accepted(t)
-    assert not _check(t)
+    assert not _check(t, 2)
     t = d.type.reshape(d.shape).copy()
     t[0, 3, 3] = domains.NODE_IN
-    assert not _check(t)
+    assert not _check(t, 0)
