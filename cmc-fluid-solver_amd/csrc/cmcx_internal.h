@@ -112,6 +112,7 @@ struct FusedWork {
     int maxn = 0;                // longest line along the axis over the grids
     int kmax = 1;                // X/Y: tiles per work item
     int vec_ok = 1;              // Z: every line starts on a 16-byte boundary
+    int nz_gcd = 0;              // Z: greatest common divisor of the grids' nz
     void release();
 };
 

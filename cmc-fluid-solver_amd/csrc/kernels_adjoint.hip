@@ -212,55 +212,6 @@ __global__ void __launch_bounds__(MAXT) adj_solve_kernel(
     }
 }
 
-// Loads of one lane's run of M rows at c0 (the first `valid` of them inside
-// the line), all issued whatever `valid` is: rows past it read from node 0
-// (addresses masked, not selected, so that no load becomes conditional).
-// `vall`: every run of the launch is whole and 16-byte aligned (vector loads).
-template <class T, int M, bool VALL>
-__device__ __forceinline__ void load_run_raw(const T* __restrict__ p, int64_t c0, int valid, T* r) {
-    constexpr int VW = Vec16<T>::n;
-    if constexpr (M % VW == 0 && VALL) {
-        {
-            using V = typename Vec16<T>::type;
-            const V* q = reinterpret_cast<const V*>(p + (c0 & -(int64_t)(valid > 0)));
-#pragma unroll
-            for (int k = 0; k < M / VW; k++) {
-                const V v = q[k];
-                const T* e = reinterpret_cast<const T*>(&v);
-#pragma unroll
-                for (int i = 0; i < VW; i++) r[k * VW + i] = e[i];
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < M; j++) r[j] = p[(c0 + j) & -(int64_t)(j < valid)];
-}
-
-template <int M, bool VALL>
-__device__ __forceinline__ void load_bytes_raw(const uint8_t* __restrict__ p, int64_t c0, int valid,
-                                               int* r) {
-    if constexpr ((M == 4 || M == 8) && VALL) {
-        {
-            const uint32_t* q = reinterpret_cast<const uint32_t*>(p + (c0 & -(int64_t)(valid > 0)));
-#pragma unroll
-            for (int k = 0; k < M / 4; k++) {
-                const uint32_t v = q[k];
-#pragma unroll
-                for (int i = 0; i < 4; i++) r[4 * k + i] = (v >> (8 * i)) & 0xff;
-            }
-            return;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < M; j++) r[j] = p[(c0 + j) & -(int64_t)(j < valid)];
-}
-
-template <class T>
-__device__ __forceinline__ void pin(const T& v) {
-    asm volatile("" ::"v"(v));
-}
-
 // Transposed line solve along z in the contiguous layout of the forward Z
 // sweep: 32 lanes per line, lane l owns rows [l*M, l*M+M) (16-byte vector
 // accesses), two lines per wavefront, kZWaves wavefronts per work-group.  The

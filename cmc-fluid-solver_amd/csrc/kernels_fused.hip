@@ -54,6 +54,7 @@
 #include "line_solve.h"
 
 #include <algorithm>
+#include <numeric>
 #include <stdexcept>
 
 namespace cmcx {
@@ -469,14 +470,18 @@ __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
 // lane's Picard-layer values wait for the merge in LDS (16-byte slots,
 // lane-contiguous) instead of registers.  LPL = 64: a line spans a whole
 // wavefront (half the rows per lane, one line per wave, twice the waves per
-// work-group) for fewer registers per lane.
-template <class T, int M, int MINW, bool PARK, int LPL>
+// work-group) for fewer registers per lane.  VALL: every run of the launch is
+// whole and 16-byte aligned, so a lane reads its rows with vector loads from
+// a masked address whatever its row count (0 or M), and all of a lane's loads
+// go out in one batch (a load used under a condition becomes a branch that
+// waits for it).
+template <class T, int M, int MINW, bool PARK, int LPL, bool VALL>
 __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
     const T* __restrict__ gv0, const T* __restrict__ gv1, const T* __restrict__ gv2,
     const T* __restrict__ gv3, Layer4<T> in, Layer4<T> tin, Layer4<T> out, Layer4<T> tout,
     int merges, int bsync, int stores, const int32_t* __restrict__ lines,
-    const int32_t* __restrict__ winst, int vec_ok) {
+    const int32_t* __restrict__ winst) {
     extern __shared__ char smem_raw[];
     const T* gv[4] = {gv0, gv1, gv2, gv3};
     const int lane = threadIdx.x & 63;
@@ -504,13 +509,70 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     const int r0 = ll * M;
     const bool ok = live && ll < used;
     const int valid = ok ? min(M, n - r0) : 0;
-    const bool vec = vec_ok && valid == M;
+    const bool vec = VALL && valid == M;
     const int64_t c0 = base + r0;
+
+    // inputs: node types (with the BC kinds), the node values when the state
+    // is out of sync with them, the state (in), the Picard layer (tin) and the
+    // neighbour lines of w (cross-line derivatives; the active lines have x, y
+    // in [1, n-2], so those lines exist)
+    int tyr[M];
+    T gb[4][M], cu[4][M], tv[4][M], wa[M], wb[M], wc[M], wd[M];
+    {
+        const T* tw = tin.f[2];
+        if constexpr (VALL) {
+            load_bytes_raw<M, true>(type, c0, valid, tyr);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                load_run_raw<T, M, true>(in.f[v], c0, valid, cu[v]);
+                load_run_raw<T, M, true>(tin.f[v], c0, valid, tv[v]);
+            }
+            load_run_raw<T, M, true>(tw + (ok ? g.sx : 0), c0, valid, wa);
+            load_run_raw<T, M, true>(tw - (ok ? g.sx : 0), c0, valid, wb);
+            load_run_raw<T, M, true>(tw + (ok ? g.sy : 0), c0, valid, wc);
+            load_run_raw<T, M, true>(tw - (ok ? g.sy : 0), c0, valid, wd);
+            if (!bsync) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) load_run_raw<T, M, true>(gv[v], c0, valid, gb[v]);
+            }
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                pin(tyr[j]);
+                pin(wa[j]);
+                pin(wb[j]);
+                pin(wc[j]);
+                pin(wd[j]);
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    pin(cu[v][j]);
+                    pin(tv[v][j]);
+                }
+            }
+            if (!bsync) {
+#pragma unroll
+                for (int j = 0; j < M; j++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) pin(gb[v][j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < M; j++) tyr[j] = j < valid ? type[c0 + j] : kOutside;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                load_run<T, M>(in.f[v] + c0, cu[v], vec, valid);
+                load_run<T, M>(tin.f[v] + c0, tv[v], vec, valid);
+            }
+            load_run<T, M>(tw + c0 + (ok ? g.sx : 0), wa, vec, valid);
+            load_run<T, M>(tw + c0 - (ok ? g.sx : 0), wb, vec, valid);
+            load_run<T, M>(tw + c0 + (ok ? g.sy : 0), wc, vec, valid);
+            load_run<T, M>(tw + c0 - (ok ? g.sy : 0), wd, vec, valid);
+        }
+    }
 
     // node types of the chunk and of the rows either side (from the neighbour lanes)
     uint8_t ty[M + 2];
 #pragma unroll
-    for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? type[c0 + j] : kOutside;
+    for (int j = 0; j < M; j++) ty[j + 1] = j < valid ? (uint8_t)tyr[j] : kOutside;
     {
         int prev = lane_prev((int)ty[M]);
         int next = lane_next((int)ty[1]);
@@ -523,42 +585,32 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
     // Dirichlet values of the boundary rows (ApplyBC0/1; the BC kinds are in
     // the flags bytes): the Picard layer while the state is in sync with the
     // node values (it holds them off NODE_IN), else the node values of the
-    // whole chunk in one vector load per field, for lanes whose chunk has such
-    // a row (a load per boundary row would sit inside the elimination chain)
-    T gb[4][M];
+    // whole chunk, one vector load per field (a load per boundary row would
+    // sit inside the elimination chain)
+    if constexpr (!VALL) {
 #pragma unroll
-    for (int j = 0; j < M; j++)
+        for (int j = 0; j < M; j++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) gb[v][j] = 0;
-    if (!bsync) {
+            for (int v = 0; v < 4; v++) gb[v][j] = 0;
         bool anyb = false;
 #pragma unroll
         for (int j = 0; j < M; j++) anyb = anyb || (role[j] != RL_IN && role[j] != RL_NONE);
-        if (anyb) {
+        if (!bsync && anyb) {
 #pragma unroll
             for (int v = 0; v < 4; v++) load_run<T, M>(gv[v] + c0, gb[v], vec, valid);
         }
     }
-
-    // inputs: state (in), Picard layer (tin) and the cross-line derivatives of w
-    T cu[4][M], tv[4][M], wx[M], wy[M];
+    if (bsync) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        load_run<T, M>(in.f[v] + c0, cu[v], vec, valid);
-        load_run<T, M>(tin.f[v] + c0, tv[v], vec, valid);
+        for (int j = 0; j < M; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) gb[v][j] = 0;
     }
-    {
-        // the active lines have x, y in [1, n-2], so the neighbour lines exist
-        T a[M], b[M];
-        const T* tw = tin.f[2];
-        load_run<T, M>(tw + c0 + (ok ? g.sx : 0), a, vec, valid);
-        load_run<T, M>(tw + c0 - (ok ? g.sx : 0), b, vec, valid);
+    T wx[M], wy[M];
 #pragma unroll
-        for (int j = 0; j < M; j++) wx[j] = a[j] - b[j];
-        load_run<T, M>(tw + c0 + (ok ? g.sy : 0), a, vec, valid);
-        load_run<T, M>(tw + c0 - (ok ? g.sy : 0), b, vec, valid);
-#pragma unroll
-        for (int j = 0; j < M; j++) wy[j] = a[j] - b[j];
+    for (int j = 0; j < M; j++) {
+        wx[j] = wa[j] - wb[j];
+        wy[j] = wc[j] - wd[j];
     }
     T tprev[4], tnext[4];
 #pragma unroll
@@ -758,10 +810,15 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
         const int vw = 16 / (int)sizeof(T);
         const int nthreads = 64 * kZWaves * (lpl / 32);
         const size_t lds = park ? (size_t)(nthreads / 64) * 64 * 4 * ((m + vw - 1) / vw) * 16 : 0;
-#define LZ(MM, W, PK, LL)                                                                          \
-    hipLaunchKernelGGL((fused_z_kernel<T, MM, W, PK, LL>), dim3(blocks), dim3(nthreads), lds, st, \
-                       insts, three_dt, tflags, gval[0], gval[1], gval[2], gval[3], in, tin, out,   \
-                       tout, merges, bsync, stores, w.d_items, w.d_inst, w.vec_ok)
+        // whole aligned runs: every line 16-byte aligned and a multiple of m rows
+        const bool vall = w.vec_ok && w.nz_gcd % m == 0;
+#define LZV(MM, W, PK, LL, VA)                                                                       \
+    hipLaunchKernelGGL((fused_z_kernel<T, MM, W, PK, LL, VA>), dim3(blocks), dim3(nthreads), lds, \
+                       st, insts, three_dt, tflags, gval[0], gval[1], gval[2], gval[3], in, tin,   \
+                       out, tout, merges, bsync, stores, w.d_items, w.d_inst)
+#define LZ(MM, W, PK, LL)              \
+    if (vall) LZV(MM, W, PK, LL, true); \
+    else LZV(MM, W, PK, LL, false)
 #define LZM(W, PK)                      \
     if (!wide) {                        \
         if (m == 2) LZ(2, W, PK, 32);   \
@@ -784,6 +841,7 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
         }
 #undef LZM
 #undef LZ
+#undef LZV
     } else {
         const int lpg = variant_lpg<T>(variant);
         const int mch = variant_chunk<T>(variant);
@@ -864,6 +922,7 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
     std::vector<int32_t> items, winst;
     w.maxn = 0;
     w.vec_ok = 1;
+    w.nz_gcd = 0;
     w.kmax = 1;
     for (const Geom& g : geoms)
         w.maxn = std::max(w.maxn, axis == AX_Z ? g.nz : axis == AX_X ? g.nx : g.ny);
@@ -873,6 +932,7 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
             const Geom& g = geoms[i];
             const uint8_t* t = h_type + offs[i];
             if (((size_t)g.nz * elem) % 16 || ((size_t)offs[i] * elem) % 16) w.vec_ok = 0;
+            w.nz_gcd = std::gcd(w.nz_gcd, g.nz);
             const int K = std::max(1, cap / g.nz);
             const size_t first = items.size();
             int run0 = -1, cnt = 0;
