@@ -984,6 +984,54 @@ class Solver final : public ISolver {
                 }
                 variant[a] = bestv;
             }
+            // Second pass, in context: a sweep timed alone leaves the caches
+            // unlike a step does (the Picard layer parked in LDS, for one, wins
+            // alone and loses between the other sweeps), so each axis's
+            // variants are timed again over whole steps (2 Picard iterations:
+            // both store forms of the X sweep), the other axes at their current
+            // choice.  A step reads cur only; next is restored afterwards (its
+            // NODE_OUT values may be rewritten by whole-run stores).
+            {
+                T* save = nullptr;
+                CMCX_HIP(hipMalloc((void**)&save, sizeof(T) * 4 * NT));
+                Layer4<T> sv;
+                for (int f = 0; f < 4; f++) sv.f[f] = save + (int64_t)f * NT;
+                copy_layer(sv, next);
+                const bool short_prep = layers_ok;
+                try {
+                    for (int a : {AX_X, AX_Y, AX_Z}) {
+                        if (plain_axis(a)) continue;
+                        float best = 1e30f;
+                        int bestv = variant[a];
+                        for (int v = 0; v < (a == AX_Z ? kZVariants : kFusedVariants); v++) {
+                            if (a != AX_Z && !variant_fits(a, v)) continue;
+                            ensure_work(a, a == AX_Z ? 0 : v);
+                            variant[a] = v;
+                            enqueue_step(dt, 2, 1, short_prep);  // warm-up
+                            CMCX_HIP(hipEventRecord(e0, st));
+                            for (int r = 0; r < reps; r++) enqueue_step(dt, 2, 1, short_prep);
+                            CMCX_HIP(hipEventRecord(e1, st));
+                            CMCX_HIP(hipEventSynchronize(e1));
+                            float ms = 0.f;
+                            CMCX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                            if (ms < best) {
+                                best = ms;
+                                bestv = v;
+                            }
+                        }
+                        variant[a] = bestv;
+                        ensure_work(a, a == AX_Z ? 0 : bestv);
+                    }
+                } catch (...) {
+                    copy_layer(next, sv);
+                    hipStreamSynchronize(st);
+                    hipFree(save);
+                    throw;
+                }
+                copy_layer(next, sv);
+                CMCX_HIP(hipStreamSynchronize(st));
+                CMCX_HIP(hipFree(save));
+            }
             CMCX_HIP(hipEventDestroy(e0));
             CMCX_HIP(hipEventDestroy(e1));
             tune_cache()[tune_key()] = {variant[0], variant[1], variant[2]};
