@@ -94,9 +94,11 @@ def _compare(d, dtype, path, variants, steps=2, tol=None):
         s.close()
 
 
-@pytest.mark.parametrize("dtype,nz", [("f32", 512), ("f64", 256)])
+@pytest.mark.parametrize("dtype,nz", [("f32", 512), ("f32", 256), ("f64", 256)])
 def test_longest_z_lines(dtype, nz):
-    """Every Z variant at the longest z lines the fused path takes."""
+    """Every Z variant at the longest z lines the fused path takes, and in FP32
+    at nz = 256 (the 256^3 workload's lines: 4 rows per lane over 64 lanes or 8
+    over 32, whole 16-byte runs)."""
     d = _long((10, 9, nz), 3)
     say(f"[long z] {dtype} nz={nz}")
     _compare(d, dtype, FUSED, [(0, 0, z) for z in range(8)])
