@@ -173,6 +173,7 @@ class Solver final : public ISolver {
         hipStream_t fs = nullptr;  // stream of the recompute, beside the reverse on `st`
         T* work = nullptr;     // xbar, tbar, state-adjoint sum, lam, node-value gradient, final state
         double* pp = nullptr;  // per-block parameter partials
+        uint16_t* nmask = nullptr;  // node words of the pointwise kernel
         int32_t* tiles[3] = {nullptr, nullptr, nullptr};
         int64_t ntiles[3] = {0, 0, 0};
         int ltiles[3] = {0, 0, 0};
@@ -342,6 +343,7 @@ class Solver final : public ISolver {
         if (adj.fs) hipStreamDestroy(adj.fs);
         hipFree(adj.work);
         hipFree(adj.pp);
+        hipFree(adj.nmask);
         for (auto* t : adj.tiles) hipFree(t);
         for (auto& e : ev_pool) {
             hipEventDestroy(e.first);
@@ -1135,22 +1137,25 @@ class Solver final : public ISolver {
         adj.steps = steps;
         adj.every = every;
         ensure_tape(true);
-        // the last interval is recorded on the way (its layers go to the tape
-        // instead of the step buffers), so the backward pass recomputes only
-        // the intervals before it
-        const int last0 = (nck - 1) * every;
+        // the last intervals, one per tape buffer, are recorded on the way
+        // (their layers go to the tape instead of the step buffers), so the
+        // backward pass recomputes only the intervals before them, and starts
+        // with two intervals to reverse while the third-last is recomputed
+        const int nrec = std::min(nck, adj.nbuf);
+        const int first0 = (nck - nrec) * every;
         const int64_t per = 1 + 6 * (int64_t)G;
         adj.holds[0] = adj.holds[1] = -1;
         for (int i = 0; i < steps; i++) {
             if (i % every == 0) copy_layer(lay(adj.ck, i / every), cur);
-            if (i >= last0) {
-                record_step(tape_buf((nck - 1) % adj.nbuf) + (i - last0) * per * 4 * NT);
+            if (i >= first0) {
+                const int sgm = i / every;
+                record_step(tape_buf(sgm % adj.nbuf) + (i - sgm * every) * per * 4 * NT);
             } else {
                 step_keeping_out_fill(true, dtt, G, 1);
                 std::swap(cur, next);
             }
         }
-        adj.holds[(nck - 1) % adj.nbuf] = nck - 1;
+        for (int sgm = nck - nrec; sgm < nck; sgm++) adj.holds[sgm % adj.nbuf] = sgm;
         adj.recorded = true;
         adj.generation++;
         adj.invalid_why.clear();
@@ -1244,7 +1249,8 @@ class Solver final : public ISolver {
                 launch_adj_sweep<T>(g, a, sp, d_type, d_bcv, d_bct, tin, out, xs, xscale, XB, TB, CB,
                                     LB, GB, alpha, beta, q == 0 ? 1 : 0,
                                     (q == 0 && it == G - 1) ? 1 : 0, (q == 0 && it == 0) ? 1 : 0,
-                                    adj.tiles[a], adj.ntiles[a], adj.ltiles[a], adj.pp, st);
+                                    adj.tiles[a], adj.ntiles[a], adj.ltiles[a], adj.pp, adj.nmask,
+                                    st);
             }
     }
 
@@ -1263,6 +1269,7 @@ class Solver final : public ISolver {
         if (!adj.work) {
             CMCX_HIP(hipMalloc((void**)&adj.work, sizeof(T) * 4 * NT * 6));
             CMCX_HIP(hipMalloc((void**)&adj.pp, sizeof(double) * 4 * adj_point_blocks()));
+            CMCX_HIP(hipMalloc((void**)&adj.nmask, sizeof(uint16_t) * g.N));
             for (int a = 0; a < 3; a++)
                 build_adj_tiles(h_type.data(), g, a, adj_lpg<T>(g), &adj.tiles[a], &adj.ntiles[a],
                                 &adj.ltiles[a]);
@@ -1274,6 +1281,8 @@ class Solver final : public ISolver {
         const bool synced0 = bv_synced, layers0 = layers_ok;
         CMCX_HIP(hipMemsetAsync(adj.work, 0, sizeof(T) * 4 * NT * 5, st));
         CMCX_HIP(hipMemsetAsync(adj.pp, 0, sizeof(double) * 4 * adj_point_blocks(), st));
+        // the BC kinds may have changed since the last backward (set_boundary)
+        launch_adj_mask(g, d_type, d_bcv, d_bct, adj.nmask, st);
         if (seed)
             for (int f = 0; f < 4; f++)
                 if (seed[f])

@@ -173,7 +173,12 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
                       Layer4<T> xs, T xscale, Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc,
                       Layer4<T> lam, Layer4<T> gb, T alpha, T beta, int zmode, int first_z, int last,
-                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp, hipStream_t st);
+                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp,
+                      const uint16_t* nmask, hipStream_t st);
+// Node words of the pointwise adjoint kernel (type, free BC kinds, NODE_IN
+// neighbours), built once per backward pass: N 16-bit words.
+void launch_adj_mask(const Geom& g, const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
+                     uint16_t* mask, hipStream_t st);
 // dst <- src, four fields of n elements, one launch.
 template <class T>
 void launch_copy4(Layer4<T> dst, Layer4<T> src, int64_t n, hipStream_t st);

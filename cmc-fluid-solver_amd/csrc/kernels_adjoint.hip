@@ -355,6 +355,36 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
     }
 }
 
+// Node word of the pointwise kernel, constant over a backward pass (built by
+// adj_mask_kernel at its start): bits 0-1 the node type, bit 2 / 3 the
+// velocity / temperature BC is free, bit 4 + nb the neighbour q - e_o (nb =
+// 2 o) or q + e_o (nb = 2 o + 1) lies inside the grid and is NODE_IN.  One
+// 16-bit load instead of nine byte loads per node.
+constexpr int kMaskFreeV = 4, kMaskFreeT = 8, kMaskNb = 4;
+
+__global__ void __launch_bounds__(256) adj_mask_kernel(Geom g, const uint8_t* __restrict__ type,
+                                                       const uint8_t* __restrict__ bcv,
+                                                       const uint8_t* __restrict__ bct,
+                                                       uint16_t* __restrict__ mask) {
+    const int sx = (int)g.sx, sy = (int)g.sy;
+    const int so[3] = {sx, sy, 1};
+    const int dim[3] = {g.nx, g.ny, g.nz};
+    const uint32_t N = (uint32_t)g.N;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < N; q += gridDim.x * blockDim.x) {
+        const int ux = (int)(q / (uint32_t)sx), rem = (int)q - ux * sx, uy = rem / sy;
+        const int pos[3] = {ux, uy, rem - uy * sy};
+        uint32_t m = type[q] & 3u;
+        if (bcv[q] == BC_FREE) m |= kMaskFreeV;
+        if (bct[q] == BC_FREE) m |= kMaskFreeT;
+        for (int nb = 0; nb < 6; nb++) {
+            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
+            if ((unsigned)(pos[o] + k) < (unsigned)dim[o] && type[(int)q + k * so[o]] == NODE_IN)
+                m |= 1u << (kMaskNb + nb);
+        }
+        mask[q] = (uint16_t)m;
+    }
+}
+
 // Pointwise part of a reversed sweep (see the file comment).  A fixed grid, so
 // that block b always owns partial-sum slot b.  Each XCD (blocks b mod 8)
 // takes one contiguous eighth of the nodes and its blocks sweep it together,
@@ -377,9 +407,7 @@ __global__ void __launch_bounds__(64 * kZWaves) adj_z_kernel(
 // RHS is a node value: Dirichlet ends and shared rows) is added here too.
 template <class T, int AXIS>
 __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p, T inv_h2,
-                                                        const uint8_t* __restrict__ type,
-                                                        const uint8_t* __restrict__ bcv,
-                                                        const uint8_t* __restrict__ bct,
+                                                        const uint16_t* __restrict__ nmask,
                                                         Layer4<T> tin, Layer4<T> out, Layer4<T> lam,
                                                         Layer4<T> tb, Layer4<T> xb, Layer4<T> cacc,
                                                         Layer4<T> gb, int last, T alpha,
@@ -415,15 +443,7 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
             const int off = k1 * so[o1] + k2 * so[o2];
             return (uint32_t)((int)q + (off & -(int)in));
         };
-        const int tq = type[q];
-        // types of the rows q - e_o (nb even) and q + e_o (nb odd), o = nb / 2
-        int traw[6];
-#pragma unroll
-        for (int nb = 0; nb < 6; nb++) {
-            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
-            traw[nb] = type[at(o, k)];
-        }
-        const int bvq = bcv[q], btq = bct[q];
+        const int mq = nmask[q];  // node type, BC kinds, NODE_IN neighbours
         // node-value gradient and state-adjoint accumulator, read whatever the
         // node types (their updates below are conditional)
         T gbq[4], caq[4];
@@ -473,9 +493,9 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         }
         // every value above is used here, once, all together: a value used
         // under a condition only would be loaded there, behind its own wait
-        asm volatile("" ::"v"(tq), "v"(bvq), "v"(btq));
+        asm volatile("" ::"v"(mq));
 #pragma unroll
-        for (int nb = 0; nb < 6; nb++) asm volatile("" ::"v"(traw[nb]), "v"(kr[nb]));
+        for (int nb = 0; nb < 6; nb++) asm volatile("" ::"v"(kr[nb]));
 #pragma unroll
         for (int f = 0; f < 4; f++)
             asm volatile("" ::"v"(tbq[f]), "v"(l[f]), "v"(xm[f]), "v"(x0[f]), "v"(xp[f]),
@@ -500,13 +520,12 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
                 asm volatile("" ::"v"(ra[nb][o]));
             }
         }
+        const int tq = mq & 3;
         const bool inq = tq == NODE_IN;
+        // the rows either side: NODE_IN or not (kOutside stands for any other)
         uint8_t tn[6];
 #pragma unroll
-        for (int nb = 0; nb < 6; nb++) {
-            const int o = nb >> 1, k = (nb & 1) ? 1 : -1;
-            tn[nb] = (unsigned)(pos[o] + k) < (unsigned)dim[o] ? (uint8_t)traw[nb] : kOutside;
-        }
+        for (int nb = 0; nb < 6; nb++) tn[nb] = (mq >> (kMaskNb + nb)) & 1 ? (uint8_t)NODE_IN : kOutside;
         const bool doxg = last && (tq == NODE_BOUND || tq == NODE_VALVE);
 
         T acc[4];
@@ -556,7 +575,7 @@ __global__ void __launch_bounds__(256) adj_point_kernel(Geom g, SweepParams<T> p
         const int rl = row_role((uint8_t)tq, tn[2 * A], tn[2 * A + 1]);
         const bool bnd = rl != RL_IN && rl != RL_NONE;
         const bool ends = rl == RL_START || rl == RL_END;
-        const bool gv = bnd && (!ends || bvq != BC_FREE), gt = bnd && (!ends || btq != BC_FREE);
+        const bool gv = bnd && (!ends || !(mq & kMaskFreeV)), gt = bnd && (!ends || !(mq & kMaskFreeT));
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             const bool gl = f < 3 ? gv : gt;
@@ -676,6 +695,14 @@ void build_adj_tiles(const uint8_t* t, const Geom& g, int axis, int lpg, int32_t
 
 int adj_point_blocks() { return kPointBlocks; }
 
+void launch_adj_mask(const Geom& g, const uint8_t* type, const uint8_t* bcv, const uint8_t* bct,
+                     uint16_t* mask, hipStream_t st) {
+    const int64_t blocks = std::min<int64_t>((g.N + 255) / 256, 8192);
+    hipLaunchKernelGGL(adj_mask_kernel, dim3((unsigned)blocks), dim3(256), 0, st, g, type, bcv, bct,
+                       mask);
+    CMCX_LAUNCH_CHECK();
+}
+
 namespace {
 
 // Grid of adj_point: the blocks that are resident at once on this device at
@@ -707,7 +734,8 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
                       const uint8_t* bcv, const uint8_t* bct, Layer4<T> tin, Layer4<T> out,
                       Layer4<T> xs, T xscale, Layer4<T> xb, Layer4<T> tb, Layer4<T> cacc,
                       Layer4<T> lam, Layer4<T> gb, T alpha, T beta, int zmode, int first_z, int last,
-                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp, hipStream_t st) {
+                      const int32_t* tiles, int64_t ntiles, int ltiles, double* pp,
+                      const uint16_t* nmask, hipStream_t st) {
     const int lpg = adj_lpg<T>(g);
     if (axis == AX_Z) {
         if (ntiles > 0) {
@@ -733,7 +761,17 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
         }
     } else if (ntiles > 0) {
         const int n = axis == AX_X ? g.nx : g.ny;
-        const int waves = adj_waves(n, lpg);  // at most 8 (adjoint_supported)
+        // at most 8 (adjoint_supported); more waves than the rows need when that
+        // lets the reduced solve take every line of the tile in one pass (a
+        // wave solves 64 / RP lines at a time, RP = the chunk count rounded up
+        // to a power of two): for 96 rows, 4 waves (one pass) instead of 3 (two)
+        int waves = adj_waves(n, lpg);
+        while (waves < 8) {
+            int rp = 1;
+            while (rp < waves * (64 / lpg)) rp <<= 1;
+            if (waves * (64 / rp) >= lpg) break;
+            waves++;
+        }
         const int P = waves * (64 / lpg);
         const size_t lds = (size_t)lpg * (kRedFields * 2 * P + 1) * sizeof(T);
 #define LA(AX, LPG)                                                                               \
@@ -751,7 +789,7 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
     // to block b (slots past the grid stay zero)
 #define LP(AX)                                                                                  \
     hipLaunchKernelGGL((adj_point_kernel<T, AX>), dim3(point_grid<T, AX>()), dim3(256), 0, st, g, p, \
-                       inv_h2, type, bcv, bct, tin, out, lam, tb, xb, cacc, gb, last, alpha, pp)
+                       inv_h2, nmask, tin, out, lam, tb, xb, cacc, gb, last, alpha, pp)
     if (axis == AX_X) LP(AX_X);
     else if (axis == AX_Y) LP(AX_Y);
     else LP(AX_Z);
@@ -766,7 +804,7 @@ void launch_adj_sweep(const Geom& g, int axis, const SweepParams<T>& p, const ui
                                       const uint8_t*, const uint8_t*, Layer4<T>, Layer4<T>,       \
                                       Layer4<T>, T, Layer4<T>, Layer4<T>, Layer4<T>, Layer4<T>,   \
                                       Layer4<T>, T, T, int, int, int, const int32_t*, int64_t,    \
-                                      int, double*, hipStream_t);
+                                      int, double*, const uint16_t*, hipStream_t);
 CMCX_ADJ_INST(float)
 CMCX_ADJ_INST(double)
 
