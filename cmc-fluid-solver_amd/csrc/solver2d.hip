@@ -19,12 +19,15 @@
 // never written (the reference reads them from freshly allocated memory).
 //
 // Kernels: one thread per (line, field) runs the Thomas recurrence of
-// Algorithms.h:21-38 in the reference's operation order (c', d' in a scratch
-// laid out (row, field, line) so neighbouring threads touch neighbouring
-// words); copies, merges and the divergence sum are elementwise.  The grids of
-// this path are small (256 x 128 in the reference configs), so the step is
-// launch-bound: each Picard iteration is 13 launches and one host read of the
-// error (the loop condition needs it).
+// Algorithms.h:21-38 in the reference's operation order.  The rows of a
+// work-group's lines are built by all its threads at once into LDS, and each
+// line's recurrence and back substitution then run from there (k2_sweep_lds;
+// lines too long for the LDS keep c', d' in a global scratch, k2_sweep), so a
+// sweep costs about one division chain per row of its longest line.  With one
+// inner iteration a direction is a copy and the sweep, which writes the
+// direction's layers itself.  Copies, merges and the divergence sum are
+// elementwise; the first G Picard iterations of a step, which run whatever
+// their error, are queued back to back and their errors read once.
 #pragma clang fp contract(off)
 
 #include "cmcx_internal.h"
@@ -94,8 +97,75 @@ __global__ void k2_clear_outer(int64_t n, const uint8_t* __restrict__ type, L3<T
     next.f[2][q] = startT;
 }
 
-// One line system per thread (blockIdx.y = field): BuildMatrix
-// (AdiSolver2D.cpp:118-178), ApplyBC0/1 (:74-116), SolveTridiagonal, UpdateSegment.
+// Row t of a line system of n rows (BuildMatrix, AdiSolver2D.cpp:118-178, and
+// ApplyBC0/1, :74-116): a, b, c, d of field `var` at cell ct.  Row 0 leaves a
+// unset (the recurrence does not read it).
+template <class T, bool DIRX>
+__device__ __forceinline__ void row2(int t, int n, int var, int64_t ct, int64_t sxs,
+                                     const uint8_t* __restrict__ bc, const L3<T>& g,
+                                     const L3<T>& lc, const L3<T>& tl, const P2& prm, T& a, T& b,
+                                     T& c, T& d) {
+    const T dt = (T)prm.dt, hx = (T)prm.dx, hy = (T)prm.dy;
+    const T v_T = (T)prm.v_T, t_phi = (T)prm.t_phi;
+    const T v_vis = (T)prm.v_vis, t_vis = (T)prm.t_vis;
+    const T vis = DIRX ? (var == 2 ? t_vis / (hx * hx) : v_vis / (hx * hx))
+                       : (var == 2 ? t_vis / (hy * hy) : v_vis / (hy * hy));
+    const T h = DIRX ? hx : hy;
+    const T* U = tl.f[0];
+    const T* V = tl.f[1];
+    const T* TT = tl.f[2];
+    if (t == 0) {
+        // ApplyBC0
+        if (bc[ct] == BC_NOSLIP) {
+            b = 1; c = 0; d = g.f[var][ct];
+        } else {
+            b = 1; c = -1; d = 0;
+        }
+    } else if (t < n - 1) {
+        const T w = DIRX ? U[ct] : V[ct];
+        a = -w / (2 * h) - vis;
+        b = 1 / dt + 2 * vis;
+        c = w / (2 * h) - vis;
+        const T lcv = lc.f[var][ct];
+        if (DIRX) {
+            if (var == 0) {
+                const T Tx = (TT[ct + sxs] - TT[ct - sxs]) / (2 * hx);
+                d = lcv / dt - v_T * Tx;
+            } else if (var == 1) {
+                d = lcv / dt;
+            } else {
+                const T ux = (U[ct + sxs] - U[ct - sxs]) / (2 * hx);
+                const T vx = (V[ct + sxs] - V[ct - sxs]) / (2 * hx);
+                const T uy = (U[ct + 1] - U[ct - 1]) / (2 * hy);
+                d = lcv / dt + t_phi * (2 * ux * ux + vx * vx + uy * vx);
+            }
+        } else {
+            if (var == 0) {
+                d = lcv / dt;
+            } else if (var == 1) {
+                const T Ty = (TT[ct + 1] - TT[ct - 1]) / (2 * hy);
+                d = lcv / dt - v_T * Ty;
+            } else {
+                const T vx = (V[ct + sxs] - V[ct - sxs]) / (2 * hx);
+                const T uy = (U[ct + 1] - U[ct - 1]) / (2 * hy);
+                const T vy = (V[ct + 1] - V[ct - 1]) / (2 * hy);
+                d = lcv / dt + t_phi * (uy * uy + 2 * vy * vy + vx * uy);
+            }
+        }
+    } else {
+        // ApplyBC1
+        c = 0;
+        if (bc[ct] == BC_NOSLIP) {
+            a = 0; b = 1; d = g.f[var][ct];
+        } else {
+            a = 1; b = -1; d = 0;
+        }
+    }
+}
+
+// One line system per thread (blockIdx.y = field): BuildMatrix, ApplyBC0/1,
+// SolveTridiagonal (Algorithms.h:21-38), UpdateSegment.  c', d' in a global
+// scratch; for systems too long for the LDS of k2_sweep_lds.
 template <class T, bool DIRX>
 __global__ void k2_sweep(int nsys, const Sys2* __restrict__ sys, int nx, int ny,
                          const uint8_t* __restrict__ bc, L3<T> g, L3<T> lc, L3<T> tl, L3<T> out,
@@ -108,73 +178,16 @@ __global__ void k2_sweep(int nsys, const Sys2* __restrict__ sys, int nx, int ny,
     auto cell = [&](int t) -> int64_t {
         return DIRX ? (int64_t)(q.a0 + t) * ny + q.a1 : (int64_t)q.a0 * ny + q.a1 + t;
     };
-    const T dt = (T)prm.dt, hx = (T)prm.dx, hy = (T)prm.dy;
-    const T v_T = (T)prm.v_T, t_phi = (T)prm.t_phi;
-    const T v_vis = (T)prm.v_vis, t_vis = (T)prm.t_vis;
-    const T vis = DIRX ? (var == 2 ? t_vis / (hx * hx) : v_vis / (hx * hx))
-                       : (var == 2 ? t_vis / (hy * hy) : v_vis / (hy * hy));
-    const T h = DIRX ? hx : hy;
-    const T* U = tl.f[0];
-    const T* V = tl.f[1];
-    const T* TT = tl.f[2];
     const int64_t sxs = ny;  // stride of i
     auto idx = [&](int t) { return ((int64_t)t * 3 + var) * nsys + s; };  // (row, field, line)
-    // row 0: ApplyBC0
-    T a, b, c, d;
-    {
-        const int64_t c0 = cell(0);
-        if (bc[c0] == BC_NOSLIP) {
-            b = 1; c = 0; d = g.f[var][c0];
-        } else {
-            b = 1; c = -1; d = 0;
-        }
-    }
+    T a = 0, b, c, d;
+    row2<T, DIRX>(0, n, var, cell(0), sxs, bc, g, lc, tl, prm, a, b, c, d);
     // forward elimination, rows built on the fly (SolveTridiagonal with c[n-1] = 0)
     T cp = c / b, dp = d / b;
     sc[idx(0)] = cp;
     sd[idx(0)] = dp;
     for (int t = 1; t < n; t++) {
-        const int64_t ct = cell(t);
-        if (t < n - 1) {
-            const T w = DIRX ? U[ct] : V[ct];
-            a = -w / (2 * h) - vis;
-            b = 1 / dt + 2 * vis;
-            c = w / (2 * h) - vis;
-            const T lcv = lc.f[var][ct];
-            if (DIRX) {
-                if (var == 0) {
-                    const T Tx = (TT[ct + sxs] - TT[ct - sxs]) / (2 * hx);
-                    d = lcv / dt - v_T * Tx;
-                } else if (var == 1) {
-                    d = lcv / dt;
-                } else {
-                    const T ux = (U[ct + sxs] - U[ct - sxs]) / (2 * hx);
-                    const T vx = (V[ct + sxs] - V[ct - sxs]) / (2 * hx);
-                    const T uy = (U[ct + 1] - U[ct - 1]) / (2 * hy);
-                    d = lcv / dt + t_phi * (2 * ux * ux + vx * vx + uy * vx);
-                }
-            } else {
-                if (var == 0) {
-                    d = lcv / dt;
-                } else if (var == 1) {
-                    const T Ty = (TT[ct + 1] - TT[ct - 1]) / (2 * hy);
-                    d = lcv / dt - v_T * Ty;
-                } else {
-                    const T vx = (V[ct + sxs] - V[ct - sxs]) / (2 * hx);
-                    const T uy = (U[ct + 1] - U[ct - 1]) / (2 * hy);
-                    const T vy = (V[ct + 1] - V[ct - 1]) / (2 * hy);
-                    d = lcv / dt + t_phi * (uy * uy + 2 * vy * vy + vx * uy);
-                }
-            }
-        } else {
-            // ApplyBC1
-            c = 0;
-            if (bc[ct] == BC_NOSLIP) {
-                a = 0; b = 1; d = g.f[var][ct];
-            } else {
-                a = 1; b = -1; d = 0;
-            }
-        }
+        row2<T, DIRX>(t, n, var, cell(t), sxs, bc, g, lc, tl, prm, a, b, c, d);
         const T den = b - a * cp;
         cp = c / den;
         dp = (d - dp * a) / den;
@@ -187,6 +200,166 @@ __global__ void k2_sweep(int nsys, const Sys2* __restrict__ sys, int nx, int ny,
         x = sd[idx(t)] - sc[idx(t)] * x;
         out.f[var][cell(t)] = x;
     }
+}
+
+// The same solve with the rows in LDS.  A work-group takes `spw` systems of
+// one field (blockIdx.y): all its threads build the rows of all its systems
+// at once into LDS ([row][system] of (a, b, c, d), and a flag byte per row),
+// then thread s runs system s's recurrence from there, c' and d' overwriting
+// c and d, and its back substitution.  The rows' global loads thus go out in
+// parallel, and the serial chain holds no memory access: its rows come from
+// LDS in register batches of kSweepBatch, each batch read while the one before
+// is eliminated (a thread's global loads would instead queue behind its own
+// scratch stores: one counter covers both on this architecture).
+// mode 0: the solution on every row into `out` (the local result layer);
+// mode 1 (a direction with one inner iteration): the solution into `lt` and
+// `out` on NODE_IN rows off the grid's last row and column, which is what the
+// direction's copies leave there (SolveDirection, AdiSolver2D.cpp:203-226:
+// the local layer takes the first solve on NODE_IN, the layer and the result
+// are copied out on NODE_IN, both copies skipping the last row and column);
+// the other rows' stores go to `dm` (the local result layer, which such a
+// direction does not read), so that no store waits on a branch.
+constexpr int kSweepLdsThreads = 256;
+constexpr int kSweepBatch = 8;
+
+template <class T>
+struct alignas(4 * sizeof(T)) Row4 {
+    T a, b, c, d;
+};
+
+template <class T, bool DIRX>
+__global__ void __launch_bounds__(kSweepLdsThreads) k2_sweep_lds(
+    int nsys, int spw, int nmax, const Sys2* __restrict__ sys, int nx, int ny,
+    const uint8_t* __restrict__ type, const uint8_t* __restrict__ bc, L3<T> g, L3<T> lc,
+    L3<T> tl, L3<T> out, L3<T> lt, L3<T> dm, int mode, P2 prm) {
+    extern __shared__ char smem2[];
+    constexpr int R = kSweepBatch;
+    const int rs = spw * nmax;  // rows in LDS
+    Row4<T>* rw = reinterpret_cast<Row4<T>*>(smem2);
+    uint8_t* sf = reinterpret_cast<uint8_t*>(rw + rs);
+    const int var = blockIdx.y;
+    const int s0 = blockIdx.x * spw;
+    const int nloc = min(spw, nsys - s0);
+    const int64_t sxs = ny;
+    auto cell = [&](const Sys2& q, int t) -> int64_t {
+        return DIRX ? (int64_t)(q.a0 + t) * ny + q.a1 : (int64_t)q.a0 * ny + q.a1 + t;
+    };
+    // rows: consecutive threads on neighbouring cells (X: the same row of
+    // neighbouring lines, Y: neighbouring rows of a line)
+    for (int e = threadIdx.x; e < rs; e += blockDim.x) {
+        const int sl = DIRX ? e % spw : e / nmax;
+        const int t = DIRX ? e / spw : e % nmax;
+        if (sl >= nloc) continue;
+        const Sys2 q = sys[s0 + sl];
+        const int n = DIRX ? q.b - q.a0 + 1 : q.b - q.a1 + 1;
+        if (t >= n) continue;
+        const int64_t ct = cell(q, t);
+        Row4<T> r;
+        r.a = 0;
+        row2<T, DIRX>(t, n, var, ct, sxs, bc, g, lc, tl, prm, r.a, r.b, r.c, r.d);
+        const int k = t * spw + sl;
+        rw[k] = r;
+        if (mode == 1) {
+            const int i = DIRX ? q.a0 + t : q.a0;
+            const int j = DIRX ? q.a1 : q.a1 + t;
+            sf[k] = type[ct] == NODE_IN && i < nx - 1 && j < ny - 1;
+        }
+    }
+    __syncthreads();
+    const int sl = threadIdx.x;
+    if (sl >= nloc) return;
+    const Sys2 q = sys[s0 + sl];
+    const int n = DIRX ? q.b - q.a0 + 1 : q.b - q.a1 + 1;
+    Row4<T>* my = rw + sl;  // row t at my[t * spw]
+    const uint8_t* mf = sf + sl;
+
+    // forward elimination (SolveTridiagonal with c[n-1] = 0); rows past the
+    // end are read clamped to the last row and not used
+    T cp, dp;
+    {
+        const Row4<T> r0 = my[0];
+        cp = r0.c / r0.b;
+        dp = r0.d / r0.b;
+        my[0].c = cp;
+        my[0].d = dp;
+    }
+    auto elim = [&](const Row4<T>& r, int t) {
+        const T den = r.b - r.a * cp;
+        cp = r.c / den;
+        dp = (r.d - dp * r.a) / den;
+        my[t * spw].c = cp;
+        my[t * spw].d = dp;
+    };
+    int t = 1;
+    Row4<T> cur[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) cur[k] = my[min(t + k, n - 1) * spw];
+    while (t + R <= n) {
+        Row4<T> nxt[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) nxt[k] = my[min(t + R + k, n - 1) * spw];
+#pragma unroll
+        for (int k = 0; k < R; k++) elim(cur[k], t + k);
+#pragma unroll
+        for (int k = 0; k < R; k++) cur[k] = nxt[k];
+        t += R;
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        if (t + k < n) elim(cur[k], t + k);
+
+    // back substitution, rows n-2 down to 0 in batches read one batch ahead
+    // (the flag bytes are read in mode 0 too, unused: no load under a branch)
+    T* const o1 = out.f[var];
+    T* const o2 = mode == 1 ? lt.f[var] : out.f[var];
+    T* const od = dm.f[var];
+    auto put = [&](int t, T v, uint8_t f) {
+        const int64_t ct = cell(q, t);
+        const bool w = mode == 0 || f != 0;
+        (w ? o1 : od)[ct] = v;
+        (w ? o2 : od)[ct] = v;
+    };
+    T x = dp;
+    put(n - 1, x, mf[(n - 1) * spw]);
+    T bc_[R], bd_[R];
+    uint8_t bf_[R];
+    t = n - 2;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int u = max(t - k, 0) * spw;
+        bc_[k] = my[u].c;
+        bd_[k] = my[u].d;
+        bf_[k] = mf[u];
+    }
+    while (t - R + 1 >= 0) {
+        T nc[R], nd[R];
+        uint8_t nf[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int u = max(t - R - k, 0) * spw;
+            nc[k] = my[u].c;
+            nd[k] = my[u].d;
+            nf[k] = mf[u];
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            x = bd_[k] - bc_[k] * x;
+            put(t - k, x, bf_[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            bc_[k] = nc[k];
+            bd_[k] = nd[k];
+            bf_[k] = nf[k];
+        }
+        t -= R;
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        if (t - k >= 0) {
+            x = bd_[k] - bc_[k] * x;
+            put(t - k, x, bf_[k]);
+        }
 }
 
 // TimeLayer2D::EvalDivError: per-block partial sums of |tx + ty| in the field
@@ -230,12 +403,18 @@ __global__ void k2_div(int nx, int ny, const uint8_t* __restrict__ type, L3<T> L
     }
 }
 
-__global__ void k2_div_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One work-group of kDiv2Blocks threads: the partials come into LDS in one
+// parallel load, then thread 0 adds them in block order.
+__global__ void __launch_bounds__(kDiv2Blocks) k2_div_finish(const double* __restrict__ part,
+                                                             int nb, double* __restrict__ out) {
+    __shared__ double sp[2 * kDiv2Blocks];
+    for (int k = threadIdx.x; k < 2 * nb; k += blockDim.x) sp[k] = part[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double s = 0, c = 0;
     for (int b = 0; b < nb; b++) {
-        s += part[2 * b];
-        c += part[2 * b + 1];
+        s += sp[2 * b];
+        c += sp[2 * b + 1];
     }
     out[0] = s;
     out[1] = c;
@@ -616,10 +795,13 @@ class Solver2D final : public ISolver2 {
     L3<T> g{}, cur{}, next{}, half{}, temp{}, tl{}, nl{};
     Sys2 *d_rows = nullptr, *d_cols = nullptr;
     int nrows = 0, ncols = 0;
+    int rows_nmax = 0, cols_nmax = 0;  // longest system of each direction
     T* scratch = nullptr;
     int64_t sc_half = 0;
     double* d_part = nullptr;
-    double* d_err = nullptr;
+    double* d_err = nullptr;  // kErrSlots (sum, count) pairs of the divergence error
+    double* h_err = nullptr;  // their pinned host copy
+    static constexpr int kErrSlots = 64;
     int method = CMCX2D_ADI;
     // Stable solver: next_w, div, q (whole grid and per-row spans), spans and
     // cell flags of the relaxation, its status words
@@ -667,7 +849,8 @@ class Solver2D final : public ISolver2 {
         if (method == CMCX2D_STABLE) build_relaxation(d->type);
         else build_systems(d->type);
         CMCX_HIP(hipMalloc((void**)&d_part, sizeof(double) * 2 * kDiv2Blocks));
-        CMCX_HIP(hipMalloc((void**)&d_err, sizeof(double) * 2));
+        CMCX_HIP(hipMalloc((void**)&d_err, sizeof(double) * 2 * kErrSlots));
+        CMCX_HIP(hipHostMalloc((void**)&h_err, sizeof(double) * 2 * kErrSlots, hipHostMallocDefault));
         if (method == CMCX2D_STABLE) {
             // StableSolver2D::Init (StableSolver2D.cpp:44-52): next and temp from
             // the state (copies that skip the last row and column)
@@ -794,6 +977,7 @@ class Solver2D final : public ISolver2 {
         hipFree(scratch);
         hipFree(d_part);
         hipFree(d_err);
+        hipHostFree(h_err);
         hipFree(d_nw);
         hipFree(d_dv);
         hipFree(d_qf);
@@ -837,6 +1021,9 @@ class Solver2D final : public ISolver2 {
             if (s.b - s.a1 + 1 < 2) throw std::string("2D: a line system shorter than 2 nodes");
         nrows = (int)rows.size();
         ncols = (int)cols.size();
+        rows_nmax = cols_nmax = 0;
+        for (const Sys2& s : rows) rows_nmax = std::max(rows_nmax, s.b - s.a0 + 1);
+        for (const Sys2& s : cols) cols_nmax = std::max(cols_nmax, s.b - s.a1 + 1);
         if (nrows) {
             CMCX_HIP(hipMalloc((void**)&d_rows, sizeof(Sys2) * nrows));
             CMCX_HIP(hipMemcpy(d_rows, rows.data(), sizeof(Sys2) * nrows, hipMemcpyHostToDevice));
@@ -858,9 +1045,44 @@ class Solver2D final : public ISolver2 {
 
     static constexpr int kAll = 0xF, kIn = 1 << NODE_IN;
 
+    // Systems per work-group of k2_sweep_lds for a direction (0: its rows do
+    // not fit the LDS): a power of two, small enough that the launch spreads
+    // over about 128 work-groups (the recurrences are serial; more systems per
+    // work-group only lengthen its row-building phase), and within 150 KiB.
+    int lds_spw(bool dirx) const {
+        const int n = dirx ? nrows : ncols, nmax = dirx ? rows_nmax : cols_nmax;
+        const int64_t per = (int64_t)nmax * (sizeof(Row4<T>) + 1);
+        int spw = 1;
+        while (spw < kSweepLdsThreads && (int64_t)(2 * spw) * per <= 150 * 1024 &&
+               (int64_t)n * 3 / (2 * spw) >= 128)
+            spw *= 2;
+        return per <= 150 * 1024 ? spw : 0;
+    }
+
+    // mode 1: k2_sweep_lds writes lt and out on the NODE_IN rows (a direction
+    // with one inner iteration), else the solution into out
+    void sweep_lds(bool dirx, int spw, int mode, L3<T> lc, L3<T> tlc, L3<T> out, L3<T> lt) {
+        const int n = dirx ? nrows : ncols, nmax = dirx ? rows_nmax : cols_nmax;
+        const size_t lds = (size_t)spw * nmax * (sizeof(Row4<T>) + 1);
+        const dim3 grid((unsigned)((n + spw - 1) / spw), 3);
+        if (dirx)
+            hipLaunchKernelGGL((k2_sweep_lds<T, true>), grid, dim3(kSweepLdsThreads), lds, st, n,
+                               spw, nmax, d_rows, nx, ny, d_type, d_bc, g, lc, tlc, out, lt, nl,
+                               mode, prm);
+        else
+            hipLaunchKernelGGL((k2_sweep_lds<T, false>), grid, dim3(kSweepLdsThreads), lds, st, n,
+                               spw, nmax, d_cols, nx, ny, d_type, d_bc, g, lc, tlc, out, lt, nl,
+                               mode, prm);
+        CMCX_LAUNCH_CHECK();
+    }
+
     void sweep(bool dirx, L3<T> lc, L3<T> lt, L3<T> out) {
         const int n = dirx ? nrows : ncols;
         if (!n) return;
+        if (const int spw = lds_spw(dirx)) {
+            sweep_lds(dirx, spw, 0, lc, lt, out, L3<T>{});
+            return;
+        }
         // sc/sd indexing uses the launch's system count as the stride
         if (dirx)
             hipLaunchKernelGGL((k2_sweep<T, true>), dim3(blocks2(n), 3), dim3(kBS2), 0, st, n,
@@ -877,6 +1099,14 @@ class Solver2D final : public ISolver2 {
     void direction(bool dirx, int L, L3<T> lc, L3<T> lt, L3<T> out) {
         if (!(dirx ? nrows : ncols)) return;  // the reference reads list[0] unconditionally
         copy(lt, tl, kAll, 0);
+        if (L == 1) {
+            if (const int spw = lds_spw(dirx)) {
+                // the solve writes what the copies below would leave on NODE_IN
+                // (the local layer is read for the stencils, lt and out are not)
+                sweep_lds(dirx, spw, 1, lc, tl, out, lt);
+                return;
+            }
+        }
         for (int it = 0; it < L; it++) {
             sweep(dirx, lc, tl, nl);
             copy(nl, tl, kIn, it == 0 ? 0 : 1);
@@ -885,16 +1115,26 @@ class Solver2D final : public ISolver2 {
         copy(nl, out, kIn, 0);
     }
 
-    double div_error(L3<T> Lr) {
+    // The divergence error of Lr into slot `slot` (sum and count), no wait.
+    void div_enqueue(L3<T> Lr, int slot) {
         hipLaunchKernelGGL((k2_div<T>), dim3(kDiv2Blocks), dim3(kBS2), 0, st, nx, ny, d_type, Lr,
                            (T)prm.dx, (T)prm.dy, d_part);
         CMCX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k2_div_finish, dim3(1), dim3(64), 0, st, d_part, kDiv2Blocks, d_err);
+        hipLaunchKernelGGL(k2_div_finish, dim3(1), dim3(kDiv2Blocks), 0, st, d_part, kDiv2Blocks,
+                           d_err + 2 * slot);
         CMCX_LAUNCH_CHECK();
-        double h[2];
-        CMCX_HIP(hipMemcpyAsync(h, d_err, sizeof(h), hipMemcpyDeviceToHost, st));
+    }
+
+    // The first n slots into h_err (waits for the stream).
+    void read_errors(int n) {
+        CMCX_HIP(hipMemcpyAsync(h_err, d_err, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, st));
         CMCX_HIP(hipStreamSynchronize(st));
-        return h[0] / h[1];
+    }
+
+    double div_error(L3<T> Lr) {
+        div_enqueue(Lr, 0);
+        read_errors(1);
+        return h_err[0] / h_err[1];
     }
 
     int update_boundaries() override {
@@ -1005,9 +1245,36 @@ class Solver2D final : public ISolver2 {
         copy(cur, next, kAll, 0);
         copy(cur, half, kAll, 0);
         copy(cur, temp, kAll, 0);
-        double err = div_error(next);
-        int it;
-        for (it = 0; it < G || err > 0.1; it++) {
+        // the error before the loop decides only whether a step with G = 0 iterates
+        double err = G == 0 ? div_error(next) : 0.0;
+        int it = 0;
+        // Iterations below G run whatever their error (only an error above 1.0
+        // ends them, with CMCX_ERR_PICARD and the state unchanged), so they are
+        // queued back to back with their errors in slots that are read once.
+        // Iterations queued after an erroring one change only working layers,
+        // which the next step's copies overwrite before anything reads them.
+        const int batch = std::min(G, (int)kErrSlots);
+        for (int k = 0; k < batch; k++) {
+            direction(true, L, cur, temp, half);
+            direction(false, L, half, temp, next);
+            div_enqueue(next, k);
+            copy(next, temp, kIn, k == 0 ? 0 : 1);
+        }
+        if (batch > 0) {
+            read_errors(batch);
+            for (int k = 0; k < batch; k++) {
+                err = h_err[2 * k] / h_err[2 * k + 1];
+                if (k > 100 || err > 0.1 * 10) {
+                    if (err_out) *err_out = err;
+                    if (iters) *iters = k + 1;
+                    set_error(k > 100 ? "2D: Picard iterations exceeded 100"
+                                      : "2D: divergence error above 1.0");
+                    return CMCX_ERR_PICARD;
+                }
+            }
+            it = batch;
+        }
+        for (; it < G || err > 0.1; it++) {
             direction(true, L, cur, temp, half);
             direction(false, L, half, temp, next);
             err = div_error(next);
