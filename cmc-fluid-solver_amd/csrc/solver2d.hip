@@ -69,6 +69,46 @@ __global__ void k2_copy(int nx, int ny, const uint8_t* __restrict__ type, int ma
     for (int f = 0; f < 3; f++) dst.f[f][q] = merge ? (dst.f[f][q] + src.f[f][q]) / 2 : src.f[f][q];
 }
 
+// Fused copies of the one-iteration path, on nodes (i < nx-1, j < ny-1) as
+// k2_copy.  k2_fan: d0..d3 <- src (the step's copies of the state, with the
+// first X direction's copy of the Picard layer).  k2_merge_fan: temp <- next
+// on NODE_IN (merge: (temp + next) / 2), then tl <- temp everywhere (the next
+// X direction's copy of the Picard layer).
+template <class T>
+__global__ void k2_fan(int nx, int ny, L3<T> src, L3<T> d0, L3<T> d1, L3<T> d2, L3<T> d3) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    const int i = (int)(q / ny), j = (int)(q % ny);
+    if (i >= nx - 1 || j >= ny - 1) return;
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        const T v = src.f[f][q];
+        d0.f[f][q] = v;
+        d1.f[f][q] = v;
+        d2.f[f][q] = v;
+        d3.f[f][q] = v;
+    }
+}
+
+template <class T>
+__global__ void k2_merge_fan(int nx, int ny, const uint8_t* __restrict__ type, int merge,
+                             L3<T> next, L3<T> temp, L3<T> tl) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)nx * ny) return;
+    const int i = (int)(q / ny), j = (int)(q % ny);
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const bool in = type[q] == NODE_IN;
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        T v = temp.f[f][q];
+        if (in) {
+            v = merge ? (v + next.f[f][q]) / 2 : next.f[f][q];
+            temp.f[f][q] = v;
+        }
+        tl.f[f][q] = v;
+    }
+}
+
 // Solver2D::UpdateBoundaries: node values into the state on BOUND/VALVE, then
 // state -> next on BOUND and VALVE (a copy that skips the last row and column).
 template <class T>
@@ -1045,6 +1085,9 @@ class Solver2D final : public ISolver2 {
 
     static constexpr int kAll = 0xF, kIn = 1 << NODE_IN;
 
+    // both directions run k2_sweep_lds with one inner iteration
+    bool one_pass(int L) const { return L == 1 && nrows && ncols && lds_spw(true) && lds_spw(false); }
+
     // Systems per work-group of k2_sweep_lds for a direction (0: its rows do
     // not fit the LDS): a power of two, small enough that the launch spreads
     // over about 128 work-groups (the recurrences are serial; more systems per
@@ -1095,10 +1138,10 @@ class Solver2D final : public ISolver2 {
         CMCX_LAUNCH_CHECK();
     }
 
-    // AdiSolver2D::SolveDirection
-    void direction(bool dirx, int L, L3<T> lc, L3<T> lt, L3<T> out) {
+    // AdiSolver2D::SolveDirection; tl_ready: tl already holds lt (a fused copy)
+    void direction(bool dirx, int L, L3<T> lc, L3<T> lt, L3<T> out, bool tl_ready = false) {
         if (!(dirx ? nrows : ncols)) return;  // the reference reads list[0] unconditionally
-        copy(lt, tl, kAll, 0);
+        if (!tl_ready) copy(lt, tl, kAll, 0);
         if (L == 1) {
             if (const int spw = lds_spw(dirx)) {
                 // the solve writes what the copies below would leave on NODE_IN
@@ -1242,9 +1285,17 @@ class Solver2D final : public ISolver2 {
         if (method == CMCX2D_STABLE) return stable_step(dt, G, err_out, iters);
         if (G < 0 || L < 1) return CMCX_ERR_ARG;
         prm.dt = dt;
-        copy(cur, next, kAll, 0);
-        copy(cur, half, kAll, 0);
-        copy(cur, temp, kAll, 0);
+        const bool fused = one_pass(L);
+        if (fused) {
+            hipLaunchKernelGGL((k2_fan<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny, cur, next,
+                               half, temp, tl);
+            CMCX_LAUNCH_CHECK();
+        } else {
+            copy(cur, next, kAll, 0);
+            copy(cur, half, kAll, 0);
+            copy(cur, temp, kAll, 0);
+        }
+        bool tl_ready = fused;  // tl holds temp (k2_fan, k2_merge_fan)
         // the error before the loop decides only whether a step with G = 0 iterates
         double err = G == 0 ? div_error(next) : 0.0;
         int it = 0;
@@ -1255,10 +1306,18 @@ class Solver2D final : public ISolver2 {
         // which the next step's copies overwrite before anything reads them.
         const int batch = std::min(G, (int)kErrSlots);
         for (int k = 0; k < batch; k++) {
-            direction(true, L, cur, temp, half);
+            direction(true, L, cur, temp, half, tl_ready);
             direction(false, L, half, temp, next);
             div_enqueue(next, k);
-            copy(next, temp, kIn, k == 0 ? 0 : 1);
+            if (fused && k + 1 < batch) {
+                hipLaunchKernelGGL((k2_merge_fan<T>), dim3(blocks2(N)), dim3(kBS2), 0, st, nx, ny,
+                                   d_type, k == 0 ? 0 : 1, next, temp, tl);
+                CMCX_LAUNCH_CHECK();
+                tl_ready = true;
+            } else {
+                copy(next, temp, kIn, k == 0 ? 0 : 1);
+                tl_ready = false;
+            }
         }
         if (batch > 0) {
             read_errors(batch);
