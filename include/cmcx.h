@@ -354,8 +354,8 @@ int cmcx2d_create_method(const cmcx2d_domain* dom, const cmcx_params* params, in
                          void* stream, cmcx2d_solver** out);
 int cmcx2d_update_boundaries(cmcx2d_solver* s);
 /* Returns CMCX_ERR_PICARD where the reference exits (AdiSolver2D.cpp:303-313),
-   with err and iters of the iteration that exited and the state as it was
-   before the step (as the reference leaves it).  iters: Picard iterations
+   with err and iters of the iteration that exited and the state as the step
+   found it (as the reference leaves it).  iters: Picard iterations
    done (may be NULL).  The iterations below num_global are queued without a
    host read between them; the host waits once for their errors. */
 int cmcx2d_step(cmcx2d_solver* s, double dt, int num_global, int num_local, double* err,
