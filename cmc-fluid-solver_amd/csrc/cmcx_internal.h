@@ -113,8 +113,18 @@ struct FusedWork {
     int kmax = 1;                // X/Y: tiles per work item
     int vec_ok = 1;              // Z: every line starts on a 16-byte boundary
     int nz_gcd = 0;              // Z: greatest common divisor of the grids' nz
+    int stream = 0;              // sweeps store past the caches (kStoreStream)
     void release();
 };
+
+// Stores bit of the fused sweeps: non-temporal stores of the output and the
+// merged Picard layer.  Set for launches whose 16 fields moved per sweep
+// exceed kStreamStoreBytes, twice the MI355X memory-side cache (256 MB): at
+// 256^3 FP32 (1.07 GB per sweep) streaming the stores takes 7.7 % (strided
+// sweeps) and 9.8 % (all three) off a step, while at 128^3 FP32 (134 MB, which
+// the cache holds) it adds 14 %.
+constexpr int kStoreStream = 4;
+constexpr int64_t kStreamStoreBytes = 512ll << 20;
 
 template <class T>
 bool fused_supported(const Geom& g);

@@ -296,19 +296,22 @@ __device__ __forceinline__ void strided_write(const Chunk<T, M>& ch, const int* 
     for (int j = 0; j < M; j++)
 #pragma unroll
         for (int v = 0; v < 4; v++) asm volatile("" ::"v"(t0[v][j]));
+    with_store_hint((stores & kStoreStream) != 0, [&](auto ntc) {
+        constexpr bool NT = decltype(ntc)::value;
 #pragma unroll
-    for (int j = 0; j < M; j++) {
-        if (role[j] == RL_NONE) continue;
-        const int c = base + (r0 + j) * s;
-        const uint8_t tf = BSYNC ? (uint8_t)0 : type[c];
+        for (int j = 0; j < M; j++) {
+            if (role[j] == RL_NONE) continue;
+            const int c = base + (r0 + j) * s;
+            const uint8_t tf = BSYNC ? (uint8_t)0 : type[c];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            T x = t0[v][j] + dd[v][j];
-            if (!BSYNC && dirichlet_row(role[j], tf, v < 3 ? 0 : 1)) x = gval[v][c];
-            if (stores & 1) out.f[v][c] = x;
-            if ((stores & 2) && role[j] == RL_IN) tout.f[v][c] = t0[v][j] + km * dd[v][j];
+            for (int v = 0; v < 4; v++) {
+                T x = t0[v][j] + dd[v][j];
+                if (!BSYNC && dirichlet_row(role[j], tf, v < 3 ? 0 : 1)) x = gval[v][c];
+                if (stores & 1) put_hint<NT>(&out.f[v][c], x);
+                if ((stores & 2) && role[j] == RL_IN) put_hint<NT>(&tout.f[v][c], t0[v][j] + km * dd[v][j]);
+            }
         }
-    }
+    });
 }
 
 // Row pitch (elements) of the LDS copy of the Picard layer: LPG lines plus
@@ -449,7 +452,8 @@ __device__ __forceinline__ void strided_tile(const InstDesc<T>* __restrict__ ins
 // values; that case, only met by a step without a boundary fill after the
 // caller replaced the state or node values, has one instantiation per tile
 // shape, without the LDS-parked layer); stores: bit 0 writes the sweep output, bit 1 the merged Picard
-// layer (a sweep whose output or Picard layer nothing reads skips it).
+// layer (a sweep whose output or Picard layer nothing reads skips it), kStoreStream streams both
+// past the caches.
 template <class T, int AXIS, int M, int LPG, int MAXT, int MINW, bool KEEP, bool BSYNC>
 __global__ void __launch_bounds__(MAXT, MINW) fused_strided_kernel(
     const InstDesc<T>* __restrict__ insts, T three_dt, const uint8_t* __restrict__ type,
@@ -686,29 +690,32 @@ __global__ void __launch_bounds__(64 * kZWaves * (LPL / 32), MINW) fused_z_kerne
             }
     }
     const T km = merges > 1 ? (T)0.75 : (T)0.5;
+    with_store_hint((stores & kStoreStream) != 0, [&](auto ntc) {
+        constexpr bool NT = decltype(ntc)::value;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        T x[M], t[M];
+        for (int v = 0; v < 4; v++) {
+            T x[M], t[M];
 #pragma unroll
-        for (int j = 0; j < M; j++) {
-            T d;  // correction of row j (xf / xl: of the chunk's first / last row)
-            if (j == 0) d = xf[v];
-            else if (j == M - 1) d = xl[v];
-            else {
-                const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
-                const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
-                d = ch.E[v][j] + F * xf[v] + G * xl[v];
+            for (int j = 0; j < M; j++) {
+                T d;  // correction of row j (xf / xl: of the chunk's first / last row)
+                if (j == 0) d = xf[v];
+                else if (j == M - 1) d = xl[v];
+                else {
+                    const T F = v < 3 ? ch.Fv[j] : ch.Ft[j];
+                    const T G = v < 3 ? ch.Gv[j] : ch.Gt[j];
+                    d = ch.E[v][j] + F * xf[v] + G * xl[v];
+                }
+                x[j] = tv[v][j] + d;
+                if (!bsync && dirichlet_row(role[j], ty[j + 1], v < 3 ? 0 : 1)) x[j] = gb[v][j];
+                // rows no segment covers get the sweep input's value
+                if (role[j] == RL_NONE) x[j] = cu[v][j];
+                t[j] = role[j] == RL_IN ? tv[v][j] + km * d : tv[v][j];
             }
-            x[j] = tv[v][j] + d;
-            if (!bsync && dirichlet_row(role[j], ty[j + 1], v < 3 ? 0 : 1)) x[j] = gb[v][j];
-            // rows no segment covers get the sweep input's value
-            if (role[j] == RL_NONE) x[j] = cu[v][j];
-            t[j] = role[j] == RL_IN ? tv[v][j] + km * d : tv[v][j];
+            // whole runs: partial-vector stores would cost far more than the bytes
+            if (stores & 1) store_run<T, M, NT>(out.f[v] + c0, x, vec, valid);
+            if (stores & 2) store_run<T, M, NT>(tout.f[v] + c0, t, vec, valid);
         }
-        // whole runs: partial-vector stores would cost far more than the bytes
-        if (stores & 1) store_run<T, M>(out.f[v] + c0, x, vec, valid);
-        if (stores & 2) store_run<T, M>(tout.f[v] + c0, t, vec, valid);
-    }
+    });
 }
 
 // Strided-sweep variants: tile width (bytes of z lines per work-group: 128 =
@@ -798,6 +805,7 @@ void launch_fused_sweep(const InstDesc<T>* insts, T three_dt, int axis, const Fu
                         int variant, hipStream_t st) {
     if (w.nblocks == 0) return;
     const unsigned blocks = (unsigned)w.nblocks;
+    if (w.stream) stores |= kStoreStream;
     if (axis == AX_Z) {
         // variants 1, 3, 5, 7 cap the registers for 4 (FP32) / 2 (FP64) waves per
         // SIMD; 2, 3, 6, 7 park the Picard layer in LDS; 4-7 spread a line over
@@ -920,6 +928,13 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
                       FusedWork& w) {
     w.release();
     std::vector<int32_t> items, winst;
+    {
+        // stream the stores of sweeps whose layers far exceed the memory-side
+        // cache (16 fields move per sweep)
+        int64_t cells = 0;
+        for (const Geom& g : geoms) cells += (int64_t)g.nx * g.ny * g.nz;
+        w.stream = (double)cells * (double)elem * 16.0 > (double)kStreamStoreBytes ? 1 : 0;
+    }
     w.maxn = 0;
     w.vec_ok = 1;
     w.nz_gcd = 0;

@@ -502,26 +502,49 @@ __device__ __forceinline__ void load_bytes(const uint8_t* __restrict__ p, uint8_
     for (int j = 0; j < M; j++) r[j] = j < valid ? p[j] : (uint8_t)0;
 }
 
-template <class T, int M>
+// A store with a cache hint: NT streams it past the caches (non-temporal),
+// for sweeps whose layers far exceed the memory-side cache, where written
+// values would only evict values that the same sweep still has to read.
+template <bool NT, class T>
+__device__ __forceinline__ void put_hint(T* p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <class T, int M, bool NT = false>
 __device__ __forceinline__ void store_run(T* __restrict__ p, const T* r, bool vec, int valid) {
     constexpr int VW = Vec16<T>::n;
     if constexpr (M % VW == 0) {
         if (vec) {
-            using V = typename Vec16<T>::type;
+            typedef T NV __attribute__((ext_vector_type(VW)));
 #pragma unroll
             for (int k = 0; k < M / VW; k++) {
-                V v;
-                T* e = reinterpret_cast<T*>(&v);
+                NV v;
 #pragma unroll
-                for (int q = 0; q < VW; q++) e[q] = r[k * VW + q];
-                reinterpret_cast<V*>(p)[k] = v;
+                for (int q = 0; q < VW; q++) v[q] = r[k * VW + q];
+                put_hint<NT>(reinterpret_cast<NV*>(p) + k, v);
             }
             return;
         }
     }
 #pragma unroll
     for (int j = 0; j < M; j++)
-        if (j < valid) p[j] = r[j];
+        if (j < valid) put_hint<NT>(p + j, r[j]);
+}
+
+// Runs f(std::true_type) when nt, else f(std::false_type): a uniform branch
+// around a whole store phase.  The empty asm statements at the ends of the
+// streaming side keep the optimiser from merging the two sides' identical
+// stores into plain ones (the hint is metadata, dropped on merging).
+template <class F>
+__device__ __forceinline__ void with_store_hint(bool nt, F&& f) {
+    if (nt) {
+        asm volatile("");
+        f(std::true_type{});
+        asm volatile("");
+    } else {
+        f(std::false_type{});
+    }
 }
 
 // Loads of one lane's run of M rows at c0 (the first `valid` of them inside
