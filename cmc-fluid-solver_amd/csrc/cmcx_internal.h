@@ -119,12 +119,12 @@ struct FusedWork {
 
 // Stores bit of the fused sweeps: non-temporal stores of the output and the
 // merged Picard layer.  Set for launches whose 16 fields moved per sweep
-// exceed kStreamStoreBytes, twice the MI355X memory-side cache (256 MB): at
-// 256^3 FP32 (1.07 GB per sweep) streaming the stores takes 7.7 % (strided
-// sweeps) and 9.8 % (all three) off a step, while at 128^3 FP32 (134 MB, which
-// the cache holds) it adds 14 %.
+// exceed kStreamStoreBytes, the MI355X memory-side cache (256 MiB).  Measured
+// on the ellipsoids workload, FP32, step time with streamed against plain
+// stores: 128^3 (134 MB per sweep, which the cache holds) +14 %, 160^3
+// (262 MB) -0.8 %, 192^3 (453 MB) -6.2 %, 256^3 (1.07 GB) -8 to -10 %.
 constexpr int kStoreStream = 4;
-constexpr int64_t kStreamStoreBytes = 512ll << 20;
+constexpr int64_t kStreamStoreBytes = 256ll << 20;
 
 template <class T>
 bool fused_supported(const Geom& g);

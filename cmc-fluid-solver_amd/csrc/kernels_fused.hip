@@ -929,7 +929,7 @@ void build_fused_work(const uint8_t* h_type, const std::vector<Geom>& geoms,
     w.release();
     std::vector<int32_t> items, winst;
     {
-        // stream the stores of sweeps whose layers far exceed the memory-side
+        // stream the stores of sweeps whose fields exceed the memory-side
         // cache (16 fields move per sweep)
         int64_t cells = 0;
         for (const Geom& g : geoms) cells += (int64_t)g.nx * g.ny * g.nz;
