@@ -537,7 +537,10 @@ struct ErrFold {
 __device__ inline void fold_cell(ErrFold& s, double qn, double qo, double tol) {
     const double d = qn - qo;
     const double a = fabs(d), m = fabs(qn), b = tol * m;
-    const bool plain = m >= 1e-290 && m <= 1.79e308 && fabs(qo) <= 1.79e308 && (b > 1e-300 || tol == 0.0);
+    // infinite or NaN values fail both tests below or decide them as the
+    // division would (an infinite change is >= tol), so only tiny |qn| and
+    // tiny tolerances need the division's exact rounding
+    const bool plain = m >= 1e-290 && (b > 1e-300 || tol == 0.0);
     bool nan = false, ge;
     if (plain && a < b * (1.0 - 1e-12)) ge = false;
     else if (plain && a > b * (1.0 + 1e-12)) ge = true;
@@ -659,14 +662,14 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
         lo[d] = ok ? spans[3 * rd + 1] : 0;
         hi[d] = ok ? spans[3 * rd + 2] : -1;
         base[d] = a[d] + 1 - lo[d];
-        b[d] = a[d] + (hi[d] - lo[d] + 1) + 1;
+        // a row beyond the grid reads element 0, the first row's zero guard
+        b[d] = ok ? a[d] + (hi[d] - lo[d] + 1) + 1 : 0;
     }
     auto clampi = [](int x, int lo_, int hi_) { return min(max(x, lo_), hi_); };
     auto inside = [&](int d, int j) -> bool { return j >= lo[d] && j <= hi[d]; };
-    auto mask = [](T v, bool in) -> double {
-        const long long m = in ? -1ll : 0ll;
-        return __longlong_as_double(__double_as_longlong((double)v) & m);
-    };
+    // A neighbour row's column j outside its span clamps onto one of that
+    // row's zero guard cells (rows beyond the grid onto element 0, the first
+    // row's guard), so the read gives the reference's q = 0 there unmasked.
     // The step's barrier: with q in LDS nothing global is shared between
     // threads within a group, so it waits for the LDS traffic only and the div
     // loads stay in flight across it.
@@ -721,7 +724,7 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
                 if (act) {
                     const int f = bf & (inside(1, jb) ? 0xff : 0);
                     if (f & PF_BOUND) {
-                        const double up = mask(bu, inside(0, jb)), dn = mask(bd, inside(2, jb));
+                        const double up = (double)bu, dn = (double)bd;
                         const double lf_ = (double)bl, rt = (double)br;
                         const double i0 = (f & PF_UP) ? up : dn, i1 = (f & PF_DOWN) ? dn : up;
                         const double j0 = (f & PF_LEFT) ? lf_ : rt, j1 = (f & PF_RIGHT) ? rt : lf_;
@@ -732,7 +735,7 @@ __global__ void __launch_bounds__(1024) k2s_poisson(
                     }
                     const int g2 = jf & (inside(1, ji) ? 0xff : 0);
                     if (g2 & PF_IN) {
-                        const double up = mask(iu, inside(0, ji)), dn = mask(id, inside(2, ji));
+                        const double up = (double)iu, dn = (double)id;
                         const double qn =
                             rcp * ((up + dn) * dy2 + ((double)il + (double)ir) * dx2 - dcur * dx2 * dy2);
                         qs[ci] = (T)qn;
