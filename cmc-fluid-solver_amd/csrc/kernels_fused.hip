@@ -23,7 +23,10 @@
 // step set-up copies BOUND and VALVE values of the state into next, and where
 // the X sweep that follows covers the node it overwrites it, here too.  The
 // strided (X, Y) sweeps store segment rows only, as the reference (storing the
-// other rows as well costs them more in bytes than it gains).
+// other rows as well costs them more in bytes than it gains).  When a launch's
+// 16 fields exceed the memory-side cache, all stores are non-temporal
+// (kStoreStream): written lines would otherwise evict lines that the sweep
+// still reads (the Picard layer for the merge, the cross-line rows).
 //
 // Each line is solved by the chunked partition method of line_solve.h (chunks
 // of M rows per lane, reduced system of 2 rows per chunk, compacted by one
